@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Throughput of the gate-application hot path on one MI355X.
+
+Workload: the 30-qubit random-layer circuit (20 layers; each layer U3(theta,
+phi, lambda) on every qubit, emitted as U1, plus CZ on a seeded perfect
+matching, emitted as CP pi; 900 gates, seed 2) applied gate by gate to a
+complex128 state in HBM (16 GiB) from |0...0>.  One step = the whole 900-gate
+circuit.  The same circuit is then run on a complex64 state (8 GiB) and its
+infidelity against the complex128 result is reported.
+
+Prints one JSON line (rank 0).  value = complex128 gates/s over the K timed
+steps; roofline = the dominant kernel (dense one-qubit sweep), timed with HIP
+events on the engine's stream during the timed steps; cpu_baseline = the
+reference's own MEM gate code (oracle/_ref) or the oracle port, on this host,
+on a bounded sample of the same circuit (its first layer, 45 gates at 30 qubits).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "30q random circuit: gates/s, HBM GB/s; fused c64 vs plain c128 @1e-6 infid"
+HBM_PEAK_GBS = 8000.0
+N_QUBITS, LAYERS, SEED = 30, 20, 2
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=N_QUBITS)
+    ap.add_argument("--layers", type=int, default=LAYERS)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c64", action="store_true")
+    ap.add_argument("--cpu-sample-gates", type=int, default=45)
+    return ap.parse_args()
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE),
+    committed under profiles/; None when no measurement exists."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p)).get(kernel, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(text, n, sample_gates):
+    """Times the first sample_gates gates of the circuit on this host's cores."""
+    import ctypes
+    import numpy as np
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    lines = [l for l in text.splitlines() if l.strip()][:sample_gates]
+    sample = "\n".join(lines) + "\n"
+    state = np.zeros(1 << n, dtype=np.complex128)
+    state[0] = 1
+    ref = os.path.join(ROOT, "oracle", "_ref", "libsqref.so")
+    secs = ctypes.c_double(0)
+    if os.path.exists(ref):
+        L = ctypes.CDLL(ref)
+        L.sqref_set_threads.argtypes = [ctypes.c_int]
+        L.sqref_time_plain_text.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                            ctypes.POINTER(ctypes.c_double)]
+        L.sqref_set_threads(threads)
+        rc = L.sqref_time_plain_text(state.ctypes.data, n, 12, sample.encode(), ctypes.byref(secs))
+        kind = "reference"
+    else:
+        L = ctypes.CDLL(os.path.join(ROOT, "oracle", "libsq_oracle.so"))
+        L.sqo_set_threads.argtypes = [ctypes.c_int]
+        L.sqo_run_plain_text.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p]
+        L.sqo_set_threads(threads)
+        t0 = time.perf_counter()
+        rc = L.sqo_run_plain_text(state.ctypes.data, n, 12, sample.encode(), None)
+        secs.value = time.perf_counter() - t0
+        kind = "port"
+    if rc != 0:
+        return None
+    del state
+    return {"value": len(lines) / secs.value, "unit": "gates/s", "cores": threads, "kind": kind,
+            "sample": f"first {len(lines)} gates (one layer) of the {n}-qubit circuit, complex128, "
+                      f"chunk_qbit=12, {threads} OpenMP threads, {secs.value:.2f} s",
+            "seconds": secs.value}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                          "(multi-GPU sharding is a design note, not an implemented path)"}))
+        return 2
+    import numpy as np
+    import subquokka_dev_amd as sq
+
+    n, layers = args.qubits, args.layers
+    text = sq.gen_circuit("random", n, layers, seed=SEED)
+    gates, _, ng, _ = sq.parse_circuit(text, n)
+    print(f"[bench] {n} qubits, {layers} layers, {ng} gates", file=sys.stderr, flush=True)
+
+    def run_steps(dtype, steps, warmup, stats):
+        eng = sq.Engine(n, dtype=dtype)
+        for _ in range(warmup):
+            eng.init_zero()
+            eng.apply_gates(gates, ng)
+        eng.sync()
+        eng.stats_reset()
+        eng.stats_enable(stats)
+        times = []
+        for _ in range(steps):
+            eng.init_zero()
+            eng.sync()
+            t0 = time.perf_counter()
+            eng.apply_gates(gates, ng)
+            eng.sync()
+            times.append(time.perf_counter() - t0)
+            print(f"[bench] {dtype} step {len(times)}: {times[-1]*1e3:.1f} ms", file=sys.stderr, flush=True)
+        st = eng.stats() if stats else {}
+        eng.stats_enable(False)
+        return eng, times, st
+
+    eng128, t128, st128 = run_steps("c128", args.steps, args.warmup, True)
+    total_t = sum(t128)
+    gates_s = args.steps * ng / total_t
+    alg_bytes = sum(v["alg_bytes"] for v in st128.values()) / args.steps
+    hbm_gbs = alg_bytes / (total_t / args.steps) / 1e9
+
+    # dominant kernel: most total time
+    dom_name, dom = max(st128.items(), key=lambda kv: kv[1]["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    bytes_per_launch = dom["alg_bytes"] / dom["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom_name),
+                "kernel": dom_name, "avg_launch_ms": round(avg_ms, 4),
+                "alg_bytes_per_launch": bytes_per_launch}
+    kernels = {k: {"launches": v["launches"], "avg_ms": round(v["total_ms"] / v["launches"], 4),
+                   "gbs": round(v["alg_bytes"] / (v["total_ms"] * 1e-3) / 1e9, 1)} for k, v in st128.items()}
+
+    out = {"metric": METRIC, "value": round(gates_s, 2), "unit": "gates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(total_t / args.steps * 1e3, 2),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded random-layer circuit, |0...0> initial state)",
+           "config": {"workload": f"random_layers_{n}q_{layers}l_seed{SEED}_plain_c128", "n_qubits": n,
+                      "layers": layers, "gates": ng, "state": "complex128 in HBM",
+                      "mode": "plain gate-by-gate (MEM_Runner::run(vector<Gate*>&) form)"},
+           "hbm_gbs": round(hbm_gbs, 1), "roofline": roofline, "kernels_c128": kernels}
+
+    if not args.no_c64:
+        eng64, t64, st64 = run_steps("c64", args.steps, args.warmup, False)
+        ov = eng128.inner(eng64)
+        infid = 1.0 - abs(ov) ** 2 / (eng128.norm2() * eng64.norm2())
+        out["c64_plain"] = {"gates_s": round(args.steps * ng / sum(t64), 2),
+                            "ms_per_step": round(sum(t64) / args.steps * 1e3, 2),
+                            "infidelity_vs_c128": infid, "max_abs_err_vs_c128": eng128.max_abs_diff(eng64)}
+        eng64.close()
+    eng128.close()
+
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cb = cpu_baseline(text, n, args.cpu_sample_gates)
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,190 @@
+/*
+ * subquokka_hip.h -- C ABI of the MI355X state-vector engine.
+ *
+ * This is the boundary a SubQuokka runner binds.  The reference selects a
+ * runner by the INI key runner_type (simulator.cpp:202-228) and calls its
+ * circuitRunner interface (simulator.hpp:85-97):
+ *     virtual void run(std::vector<Gate*>&);               plain gate-by-gate
+ *     virtual void run(std::vector<std::vector<Gate*>>&);  subcircuit blocks
+ * MEM_Runner owns a 2^N complex<double> state in RAM (MEM_Runner.cpp:57-88).
+ * Here the engine owns the state in HBM (or, for the out-of-core runner, in
+ * pinned host memory) and each entry point below replaces one piece of that
+ * path; the replaced reference code is cited on each declaration.
+ *
+ * Conventions:
+ *  - Qubit q is bit q of the amplitude index (simulator.cpp:168-175).
+ *  - Host state buffers are interleaved (re, im): complex<double> for
+ *    SQ_C128, complex<float> for SQ_C64.
+ *  - Every call returns SQ_OK (0) or a negative sq_status; no call exits the
+ *    process.  sq_last_error() returns the message of the last failure on that
+ *    engine.  Calls on one engine are serialised by the caller; separate
+ *    engines are independent (no global state).
+ *  - No torch types: plain pointers and sizes only.
+ */
+#ifndef SUBQUOKKA_HIP_H
+#define SUBQUOKKA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sq_status {
+    SQ_OK = 0,
+    SQ_ERR_ARG = -1,          /* bad argument / size mismatch */
+    SQ_ERR_UNKNOWN_GATE = -2, /* simulator.cpp:593-596 "[setGate]: Not implemented yet." */
+    SQ_ERR_QUBIT_INDEX = -3,  /* simulator.cpp:764-774 "[setGate]: Qubit Index Error" */
+    SQ_ERR_CONTRACT = -4,     /* input outside the reference's valid layout
+                                 (U2 targ[0] > targ[1], circuit.cpp:2809; repeated targets) */
+    SQ_ERR_PARSE = -5,        /* malformed circuit/INI text */
+    SQ_ERR_OOM = -6,          /* device or pinned-host allocation failed */
+    SQ_ERR_HIP = -7,          /* a HIP runtime call failed */
+    SQ_ERR_UNSUPPORTED = -8,  /* runner_type RDMA, VSWAP_3_3/4_4/6_6, U3, MPI_* */
+    SQ_ERR_IO = -9
+} sq_status;
+
+typedef enum sq_dtype {
+    SQ_C128 = 0,   /* complex<double>, the reference's only state type */
+    SQ_C64 = 1     /* complex<float> */
+} sq_dtype;
+
+/* Gate kinds: the MEM gate names of setGate_MEM (simulator.cpp:542-597). */
+typedef enum sq_gate_kind {
+    SQ_GATE_H = 1, SQ_GATE_X, SQ_GATE_Y, SQ_GATE_Z, SQ_GATE_P,
+    SQ_GATE_RX, SQ_GATE_RY, SQ_GATE_RZ, SQ_GATE_U1, SQ_GATE_U2,
+    SQ_GATE_SWAP, SQ_GATE_CP, SQ_GATE_RZZ, SQ_GATE_VSWAP_1_1, SQ_GATE_VSWAP_2_2
+} sq_gate_kind;
+
+/* One circuit line, parsed.  Replaces the Gate* objects built by setGate_MEM
+ * (simulator.cpp:259-356, 542-597; circuit.hpp:19-47).
+ *   q[]   targets as written (symmetric 2q gates may be in either order)
+ *   param angle of P, RX, RY, RZ, CP, RZZ
+ *   mat   U1: 8 reals, row-major (re,im) of [[c0,c1],[c2,c3]] (circuit.cpp:274-277)
+ *         U2: 32 reals, 4x4 row-major in basis |q0 q1> with q0 as the most
+ *             significant bit (the layout circuit.cpp:2811-2819 reverses). */
+typedef struct sq_gate {
+    int32_t kind;
+    int32_t nq;
+    int32_t q[4];
+    double param;
+    double mat[32];
+} sq_gate;
+
+/* A dense k-qubit block (1 <= k <= 6) with a 2^k x 2^k complex matrix in FP64,
+ * row-major interleaved (re,im); local index bit j <-> qubit q[j].
+ * No reference counterpart: it is the MI355X form of the subcircuit blocks
+ * that MEM_Runner::run(subcircuits) applies tile by tile (MEM_Runner.cpp:173-182). */
+typedef struct sq_fused {
+    int32_t k;
+    int32_t q[6];
+    const double *u;
+} sq_fused;
+
+typedef struct sq_config {
+    int32_t total_qbit;       /* [system] total_qbit (simulator.cpp:90) */
+    int32_t chunk_qbit;       /* [system] chunk_qbit (simulator.cpp:94); blocked-mode tile */
+    int32_t dtype;            /* sq_dtype */
+    int32_t device;           /* HIP device ordinal */
+    int32_t resident_qbit;    /* 0: whole state in HBM (MEM runner).  >0: out-of-core
+                                 runner, state in pinned host memory, 2^resident_qbit
+                                 amplitudes per streamed chunk (DIO_Runner role) */
+    int32_t reserved[3];
+} sq_config;
+
+typedef struct sq_engine sq_engine;
+
+/* ---- lifetime: replaces MEM_Runner::MEM_Runner (MEM_Runner.cpp:57-88) ---- */
+int sq_create(const sq_config *cfg, sq_engine **out);
+int sq_destroy(sq_engine *e);
+/* |0...0>: buffer[0] = 1 (MEM_Runner.cpp:79-81; setupStateFile simulator.cpp:639-658) */
+int sq_init_zero(sq_engine *e);
+/* Copies a host state in/out, converting between dtypes if they differ.
+ * n_amps must be 2^total_qbit.  (No reference counterpart beyond the public
+ * MEM_Runner::buffer, MEM_Runner.hpp:27, and the text dump simulator.cpp:696-704.) */
+int sq_set_state(sq_engine *e, const void *host, uint64_t n_amps, int32_t dtype);
+int sq_get_state(sq_engine *e, void *host, uint64_t n_amps, int32_t dtype);
+/* Device pointer and stream of the resident state (SQ_ERR_ARG for the
+ * out-of-core runner).  Lets a caller time or read the state in place. */
+int sq_device_state(sq_engine *e, void **dptr, void **hip_stream);
+int sq_synchronize(sq_engine *e);
+const char *sq_last_error(const sq_engine *e);
+
+/* ---- the hot path ---- */
+/* Plain gate-by-gate sweep: MEM_Runner::run(vector<Gate*>&) (MEM_Runner.cpp:101-134). */
+int sq_apply_gates(sq_engine *e, const sq_gate *gates, uint64_t n);
+/* Subcircuit sweep: MEM_Runner::run(vector<vector<Gate*>>&) (MEM_Runner.cpp:136-185).
+ * block_sizes[i] gates form block i; blocks whose first gate is VSWAP_* are
+ * qubit-swap passes, the others are applied tile by tile in LDS. */
+int sq_apply_blocks(sq_engine *e, const sq_gate *gates, const uint32_t *block_sizes,
+                    uint64_t n_blocks);
+/* Dense fused blocks on MFMA (no reference counterpart, see sq_fused). */
+int sq_apply_fused(sq_engine *e, const sq_fused *blocks, uint64_t n);
+/* Composes gates (all targets inside q[0..k-1]) into one 2^k x 2^k FP64 matrix on
+ * the GPU; u_out receives 2*4^k doubles. */
+int sq_compose_block(sq_engine *e, const sq_gate *gates, uint64_t n, int32_t k,
+                     const int32_t *q, double *u_out);
+
+/* ---- reductions (FP64 accumulation on the GPU) ---- */
+int sq_norm2(sq_engine *e, double *out);
+/* <psi| sum_i Z_i Z_{(i+1) mod N} |psi> */
+int sq_expect_zz_ring(sq_engine *e, double *out);
+/* <a|b> over two engines of the same size (any dtypes). */
+int sq_inner(sq_engine *a, sq_engine *b, double *re, double *im);
+/* max_i |a_i - b_i| over two engines of the same size. */
+int sq_max_abs_diff(sq_engine *a, sq_engine *b, double *out);
+
+/* ---- adjoint gradient (no reference counterpart) ----
+ * Hardware-efficient ansatz: layers x (RY(theta) on every qubit, RZ(theta) on
+ * every qubit, CNOT ring i -> (i+1) mod N); theta has 2*N*layers entries in
+ * gate order.  Energy E = <psi| sum_i Z_i Z_{i+1} |psi> from |0...0>.
+ * checkpoint_every: layers between re-anchoring checkpoints (0 = none). */
+int sq_hea_energy(sq_engine *e, int32_t layers, const double *theta, double *energy);
+int sq_adjoint_grad_hea(sq_engine *e, int32_t layers, const double *theta,
+                        int32_t checkpoint_every, double *energy, double *grad);
+
+/* ---- per-kernel timing (HIP events on the engine's stream) ---- */
+typedef struct sq_kstat {
+    char name[48];
+    uint64_t launches;
+    double total_ms;
+    double alg_bytes;   /* algorithmic bytes: 2*B*2^N*f_g summed over launches */
+} sq_kstat;
+int sq_stats_enable(sq_engine *e, int32_t on);
+int sq_stats_get(sq_engine *e, sq_kstat *out, int32_t cap, int32_t *n_out);
+int sq_stats_reset(sq_engine *e);
+
+/* ---- host-side formats (no device needed) ---- */
+/* Parses one gate line of the MEM grammar (setGate_MEM, simulator.cpp:542-597;
+ * check_targ_in_N simulator.cpp:764-774). */
+int sq_parse_gate(const char *line, int32_t total_qbit, sq_gate *out);
+/* Parses a whole circuit file text.  Plain: one gate per line.  Subcircuit
+ * (simulator.cpp:619-636): "<count>" then count gate lines, repeated; block
+ * sizes go to block_sizes (may be NULL for plain).  On overflow returns
+ * SQ_ERR_ARG with the needed counts in *n_gates / *n_blocks. */
+int sq_parse_circuit(const char *text, int32_t total_qbit, int32_t is_subcircuit,
+                     sq_gate *gates, uint64_t gate_cap, uint64_t *n_gates,
+                     uint32_t *block_sizes, uint64_t block_cap, uint64_t *n_blocks,
+                     int32_t *err_line);
+
+/* Seeded synthetic inputs (SplitMix64; identical on every host).
+ * kind: 0 = QFT in the reference's order (python_script/cir_qft.py:27-35),
+ *       1 = random layers (U3 on every qubit, emitted as U1, then CZ on a
+ *           random perfect matching, emitted as CP pi), 2 = HEA (RY, RZ, CNOT
+ *           ring as U2).  Writes circuit text into buf (needs *len bytes). */
+int sq_gen_circuit(int32_t kind, int32_t n_qubits, int32_t layers, uint64_t seed,
+                   const double *theta, char *buf, uint64_t cap, uint64_t *len);
+/* Gaussian (re, im) i.i.d., normalised in FP64: 2^n complex<double>. */
+int sq_gen_state(int32_t n_qubits, uint64_t seed, double *out);
+
+/* Planner: groups a plain circuit into blocks.  mode 0 = subcircuit text with
+ * VSWAP passes for a chunk of chunk_qbit low qubits (finder.cpp:69-333 format);
+ * mode 1 = fused k-qubit blocks (fused_k).  Output text into buf. */
+int sq_plan(const char *circuit_text, int32_t total_qbit, int32_t mode, int32_t param,
+            char *buf, uint64_t cap, uint64_t *len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SUBQUOKKA_HIP_H */

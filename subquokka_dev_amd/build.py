@@ -1,0 +1,83 @@
+"""Builds the native libraries in-tree for gfx950.
+
+    libsubquokka_hip.so   HIP engine + C++ host formats behind include/subquokka_hip.h
+    quokka                CLI with the reference's -i <ini> -c <circuit> interface
+
+Compiled with hipcc directly (no CMake, no JIT cache): the outputs sit next to
+the sources so they travel with the repository to the GPU host.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libsubquokka_hip.so")
+CLI = os.path.join(HERE, "quokka")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# -packed-fp32-ops off: hipcc is not allowed to form packed FP32 instructions.
+HIP_FLAGS = [
+    "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function", "-Wno-unused-value",
+]
+INCLUDES = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+
+HIP_SOURCES = ["sq_engine.hip", "sq_blocks.hip"]
+HOST_SOURCES = ["host/formats.cpp"]
+CLI_SOURCES = []
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def _objdir():
+    d = os.path.join(HERE, "_build")
+    os.makedirs(d, exist_ok=True)
+    return d
+
+
+def build(verbose=True, jobs=8):
+    """Compiles every source and links the shared library and the CLI."""
+    if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    objdir = _objdir()
+    procs, objs = [], []
+    for src in HIP_SOURCES + HOST_SOURCES:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        flags = list(HIP_FLAGS)
+        if src.endswith(".cpp"):
+            flags = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-x", "c++"]
+            cmd = ["/usr/bin/g++"] + flags + INCLUDES + ["-c", os.path.join(CSRC, src), "-o", obj]
+        else:
+            cmd = [HIPCC] + flags + INCLUDES + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print("+", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+        objs.append(obj)
+        if len(procs) >= jobs:
+            if procs.pop(0).wait() != 0:
+                raise RuntimeError("compile failed")
+    for p in procs:
+        if p.wait() != 0:
+            raise RuntimeError("compile failed")
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lgomp"])
+    os.replace(tmp, LIB)
+    if CLI_SOURCES:
+        srcs = [os.path.join(CSRC, s) for s in CLI_SOURCES]
+        _run(["/usr/bin/g++", "-std=c++17", "-O2", "-o", CLI] + INCLUDES + srcs +
+             ["-L" + HERE, "-lsubquokka_hip", "-Wl,-rpath,$ORIGIN", "-fopenmp"])
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,59 @@
+// engine_internal.hpp -- the engine object shared by the .hip translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "subquokka_hip.h"
+#include "host/gate_lowering.hpp"
+
+struct PendingStat {
+    char name[48];
+    double alg_bytes;
+    hipEvent_t start, stop;
+};
+
+struct sq_engine {
+    int n = 0;                  // total_qbit
+    int chunk = 12;             // chunk_qbit (tile width of the subcircuit sweep)
+    int dtype = SQ_C128;
+    int device = 0;
+    void *d_state = nullptr;    // 2^n amplitudes, interleaved (re, im)
+    double *d_part = nullptr;   // reduction partials
+    void *d_stage = nullptr;    // dtype-conversion staging (16 MiB amplitudes max)
+    hipStream_t stream = nullptr;
+    unsigned max_blocks = 2048;
+    std::string err;
+
+    bool stats_on = false;
+    std::vector<PendingStat> pending;
+
+    // extension buffers owned by other translation units
+    void *d_blockprog = nullptr;    // subcircuit program (device copy)
+    size_t blockprog_cap = 0;
+    void *d_fused = nullptr;        // fused-block matrices
+    size_t fused_cap = 0;
+    void *d_aux = nullptr;          // adjoint: second state (lambda)
+    void *d_ckpt = nullptr;         // adjoint: checkpoints
+    size_t ckpt_cap = 0;
+
+    uint64_t n_amps() const { return 1ULL << n; }
+    size_t amp_bytes() const { return dtype == SQ_C128 ? 16 : 8; }
+    size_t state_bytes() const { return (size_t)n_amps() * amp_bytes(); }
+
+    int fail(int code, const std::string &msg);
+    int hip_fail(hipError_t e, const char *what);
+    int check_launch(const char *what);
+    unsigned grid_for(uint64_t work_items, unsigned per_block) const;
+    int stat_begin(const char *name, double alg_bytes);
+    void stat_end(int slot);
+
+    int lower(const sq_gate &g, sq::LoweredGate &lg);
+    int apply_lowered(const sq::LoweredGate &g);
+    int ensure_partials();
+    int norm_zz(double *v);
+    int compare(sq_engine *o, double *v);
+    void free_extras();
+};

@@ -1,0 +1,206 @@
+"""GPU parity of the plain and blocked sweeps against the FP64 oracle.
+
+Bars (stated here, asserted below):
+  complex128: max |psi_gpu - psi_oracle| <= 1e-12 (normalised states)
+  complex64 : max |psi_gpu - psi_oracle| <= 2e-6 and infidelity <= 1e-6
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import subquokka_dev_amd as sq
+from helpers import GOLDEN, gate_line, infidelity, random_circuit, random_state
+
+pytestmark = pytest.mark.gpu
+
+TOL128 = 1e-12
+TOL64 = 2e-6
+INFID64 = 1e-6
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_gpu(n, text, state, dtype="c128", sub=False, chunk=0):
+    eng = sq.Engine(n, dtype=dtype, chunk=chunk)
+    try:
+        eng.set_state(state.astype(np.complex128 if dtype == "c128" else np.complex64))
+        gates, blocks, ng, nb = sq.parse_circuit(text, n, is_subcircuit=sub)
+        if sub:
+            eng.apply_blocks(gates, blocks, nb)
+        else:
+            eng.apply_gates(gates, ng)
+        return eng.get_state(np.complex128)
+    finally:
+        eng.close()
+
+
+def check(got, want, dtype, what=""):
+    err = float(np.max(np.abs(got - want)))
+    if dtype == "c128":
+        assert err <= TOL128, f"{what}: max abs err {err:.3e}"
+    else:
+        assert err <= TOL64, f"{what}: max abs err {err:.3e}"
+        assert infidelity(got, want) <= INFID64, what
+
+
+ONE_Q = ["H", "X", "Y", "Z", "P", "RX", "RY", "RZ", "U1"]
+TWO_Q = ["U2", "SWAP", "CP", "RZZ"]
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [4, 9, 13])
+def test_one_qubit_every_target(oracle, dtype, n):
+    # n=4: scalar fallback; 9 and 13: vector / lane-shuffle / register paths.
+    import random
+    rng = random.Random(n)
+    nrng = np.random.default_rng(n)
+    lines = [gate_line(k, [t], rng, nrng) for t in range(n) for k in ONE_Q]
+    s0 = random_state(n, 11 * n)
+    say(f"1q n={n} {dtype} gates={len(lines)}")
+    for line in lines:
+        want = oracle.run(s0, n, line + "\n")
+        got = run_gpu(n, line + "\n", s0, dtype)
+        check(got, want, dtype, line)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [4, 9, 12])
+def test_two_qubit_every_pair(oracle, dtype, n):
+    import random
+    rng = random.Random(100 + n)
+    nrng = np.random.default_rng(100 + n)
+    s0 = random_state(n, 7 * n)
+    say(f"2q n={n} {dtype}")
+    text_lines = []
+    for a in range(n):
+        for b in range(a + 1, n):
+            for k in TWO_Q:
+                text_lines.append(gate_line(k, [a, b], rng, nrng))
+    # one launch per gate through the ABI, checked after every 16 gates
+    for i in range(0, len(text_lines), 16):
+        chunk = "\n".join(text_lines[i:i + 16]) + "\n"
+        want = oracle.run(s0, n, chunk)
+        got = run_gpu(n, chunk, s0, dtype)
+        check(got, want, dtype, chunk.splitlines()[0])
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_unsorted_symmetric_targets(oracle, dtype):
+    n = 10
+    s0 = random_state(n, 3)
+    text = "CP 7 2 0.3\nRZZ 9 0 1.1\nSWAP 8 1\n"
+    want = oracle.run(s0, n, text)     # the oracle sorts symmetric targets
+    check(run_gpu(n, text, s0, dtype), want, dtype, text)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("q", [(0, 1, 6, 9), (2, 5, 7, 11), (0, 3, 10, 11), (6, 8, 9, 12)])
+def test_vswap_2_2(oracle, dtype, q):
+    n = 13
+    s0 = random_state(n, sum(q))
+    text = "1\nVSWAP_2_2 %d %d %d %d\n" % q
+    chunk = q[1] + 1
+    want = oracle.run(s0, n, text, chunk=chunk, sub=True)
+    check(run_gpu(n, text, s0, dtype, sub=True, chunk=chunk), want, dtype, text)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [7, 14, 17])
+def test_random_circuit(oracle, dtype, n):
+    text = random_circuit(n, 300, seed=n)
+    s0 = random_state(n, n + 1)
+    say(f"random n={n} {dtype}")
+    check(run_gpu(n, text, s0, dtype), oracle.run(s0, n, text), dtype, f"random n={n}")
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_config1_qft20_seeded_state(oracle, dtype):
+    # 20-qubit QFT in cir_qft.py order on the seeded Gaussian state, one MEM chunk.
+    n = 20
+    text = sq.gen_circuit("qft", n)
+    s0 = sq.gen_state(n, 1)
+    say(f"qft20 {dtype}")
+    want = oracle.run(s0, n, text, chunk=20)
+    check(run_gpu(n, text, s0, dtype), want, dtype, "qft20")
+
+
+@pytest.mark.parametrize("circ", ["qft", "qaoa", "h", "rx", "rzz"])
+def test_reference_sub_fixtures_blocked(oracle, circ):
+    n = 21
+    text = open(os.path.join(GOLDEN, "ref_circuits", f"sub_{circ}21.txt")).read()
+    s0 = random_state(n, 21)
+    say(f"sub fixture {circ}21")
+    want = oracle.run(s0, n, text, chunk=12, sub=True)
+    check(run_gpu(n, text, s0, "c128", sub=True, chunk=12), want, "c128", circ)
+
+
+def test_reductions(oracle):
+    n = 16
+    s0 = random_state(n, 2)
+    eng = sq.Engine(n)
+    try:
+        eng.set_state(s0)
+        assert abs(eng.norm2() - 1.0) < 1e-13
+        assert abs(eng.expect_zz_ring() - oracle.expect_zz_ring(s0, n)) < 1e-12
+        other = sq.Engine(n, dtype="c64")
+        other.set_state(s0.astype(np.complex64))
+        ip = eng.inner(other)
+        want = np.vdot(s0, s0.astype(np.complex64).astype(np.complex128))
+        assert abs(ip - want) < 1e-12
+        assert abs(eng.max_abs_diff(other) - np.max(np.abs(s0 - s0.astype(np.complex64)))) < 1e-15
+        other.close()
+    finally:
+        eng.close()
+
+
+def test_error_paths_leave_state_untouched():
+    n = 8
+    s0 = random_state(n, 4)
+    eng = sq.Engine(n)
+    try:
+        eng.set_state(s0)
+        gates, _, ng, _ = sq.parse_circuit("H 0\nH 1\n", n)
+        gates[1].q[0] = 9          # out of range after parsing
+        with pytest.raises(sq.SqError) as ei:
+            eng.apply_gates(gates, ng)
+        assert ei.value.code == -3
+        assert np.array_equal(eng.get_state(), s0)
+        with pytest.raises(sq.SqError):
+            eng.set_state(np.zeros(10, dtype=np.complex128))
+    finally:
+        eng.close()
+
+
+def test_large_state_round_trip_property():
+    # 30 qubits complex128 (16 GiB): a 2-layer random circuit followed by its
+    # inverse returns |0...0>; the norm stays 1.  Size-independent checks.
+    n = 30
+    text = sq.gen_circuit("random", n, 2, seed=9)
+    inv = []
+    for line in reversed(text.strip().splitlines()):
+        tok = line.split()
+        if tok[0] == "U1":
+            v = [float(x) for x in tok[2:10]]
+            u = np.array([complex(v[2 * i], v[2 * i + 1]) for i in range(4)]).reshape(2, 2).conj().T
+            inv.append(f"U1 {tok[1]} " + " ".join(f"{float(z.real)!r} {float(z.imag)!r}" for z in u.flatten()))
+        else:
+            inv.append(f"CP {tok[1]} {tok[2]} {-float(tok[3])!r}")
+    say("30q round trip")
+    eng = sq.Engine(n)
+    try:
+        eng.apply_text(text)
+        assert abs(eng.norm2() - 1) < 1e-12
+        eng.apply_text("\n".join(inv) + "\n")
+        ref = sq.Engine(n)
+        try:
+            assert eng.max_abs_diff(ref) < 1e-12
+        finally:
+            ref.close()
+    finally:
+        eng.close()
