@@ -29,8 +29,8 @@ HIP_FLAGS = [
 INCLUDES = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
 HIP_SOURCES = ["sq_engine.hip", "sq_blocks.hip"]
-HOST_SOURCES = ["host/formats.cpp"]
-CLI_SOURCES = []
+HOST_SOURCES = ["host/formats.cpp", "host/simulator.cpp", "host/planner.cpp"]
+CLI_SOURCES = ["host/quokka_main.cpp"]
 
 
 def _run(cmd):
@@ -73,8 +73,8 @@ def build(verbose=True, jobs=8):
     os.replace(tmp, LIB)
     if CLI_SOURCES:
         srcs = [os.path.join(CSRC, s) for s in CLI_SOURCES]
-        _run(["/usr/bin/g++", "-std=c++17", "-O2", "-o", CLI] + INCLUDES + srcs +
-             ["-L" + HERE, "-lsubquokka_hip", "-Wl,-rpath,$ORIGIN", "-fopenmp"])
+        _run(["/usr/bin/g++", "-std=c++17", "-O2", "-o", CLI] + INCLUDES + ["-I" + os.path.join(CSRC, "host")] +
+             srcs + ["-L" + HERE, "-lsubquokka_hip", "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 

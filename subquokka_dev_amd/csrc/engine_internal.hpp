@@ -25,6 +25,7 @@ struct sq_engine {
     void *d_stage = nullptr;    // dtype-conversion staging (16 MiB amplitudes max)
     hipStream_t stream = nullptr;
     unsigned max_blocks = 2048;
+    size_t lds_per_block = 65536;   // hipDeviceAttributeMaxSharedMemoryPerBlock
     std::string err;
 
     bool stats_on = false;
@@ -38,6 +39,7 @@ struct sq_engine {
     void *d_aux = nullptr;          // adjoint: second state (lambda)
     void *d_ckpt = nullptr;         // adjoint: checkpoints
     size_t ckpt_cap = 0;
+    uint64_t n_tile_passes = 0;     // tile passes launched (diagnostics)
 
     uint64_t n_amps() const { return 1ULL << n; }
     size_t amp_bytes() const { return dtype == SQ_C128 ? 16 : 8; }
@@ -47,6 +49,7 @@ struct sq_engine {
     int hip_fail(hipError_t e, const char *what);
     int check_launch(const char *what);
     unsigned grid_for(uint64_t work_items, unsigned per_block) const;
+    unsigned grid_flat(uint64_t work_items, unsigned per_block) const;
     int stat_begin(const char *name, double alg_bytes);
     void stat_end(int slot);
 

@@ -1,0 +1,245 @@
+// planner.cpp -- see planner.hpp.
+#include "planner.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gate_lowering.hpp"
+
+namespace sq {
+
+namespace {
+const char *kind_name(int k) {
+    static const char *names[] = {"?", "H", "X", "Y", "Z", "P", "RX", "RY", "RZ", "U1", "U2",
+                                  "SWAP", "CP", "RZZ", "VSWAP_1_1", "VSWAP_2_2"};
+    return (k >= 1 && k <= 15) ? names[k] : "?";
+}
+
+void appendf(std::string &s, const char *fmt, double v) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, fmt, v);
+    s += buf;
+}
+}  // namespace
+
+std::vector<int> gate_qubits(const sq_gate &g) {
+    return std::vector<int>(g.q, g.q + std::max(0, std::min(4, g.nq)));
+}
+
+std::string gate_to_line(const sq_gate &g) {
+    std::string s = kind_name(g.kind);
+    for (int i = 0; i < g.nq; i++) s += " " + std::to_string(g.q[i]);
+    switch (g.kind) {
+    case SQ_GATE_P: case SQ_GATE_RX: case SQ_GATE_RY: case SQ_GATE_RZ: case SQ_GATE_CP: case SQ_GATE_RZZ:
+        appendf(s, " %.17g", g.param);
+        break;
+    case SQ_GATE_U1:
+        for (int i = 0; i < 8; i++) appendf(s, " %.17g", g.mat[i]);
+        break;
+    case SQ_GATE_U2:
+        for (int i = 0; i < 32; i++) appendf(s, " %.17g", g.mat[i]);
+        break;
+    default:
+        break;
+    }
+    return s;
+}
+
+namespace {
+
+// Gates runnable with qubit set `in` from the pending list, in order.
+int count_runnable(const std::vector<sq_gate> &gates, const std::vector<int> &pending,
+                   const std::vector<char> &in, std::vector<char> &blocked, std::vector<int> *out) {
+    std::fill(blocked.begin(), blocked.end(), 0);
+    int cnt = 0;
+    for (int gi : pending) {
+        const sq_gate &g = gates[gi];
+        bool ok = true;
+        for (int i = 0; i < g.nq; i++) if (!in[g.q[i]] || blocked[g.q[i]]) { ok = false; break; }
+        if (ok) {
+            cnt++;
+            if (out) out->push_back(gi);
+        } else {
+            for (int i = 0; i < g.nq; i++) blocked[g.q[i]] = 1;
+        }
+    }
+    return cnt;
+}
+
+}  // namespace
+
+std::vector<PlanPass> plan_passes(const std::vector<sq_gate> &gates, int n, int budget,
+                                  const std::vector<int> &fixed) {
+    std::vector<PlanPass> passes;
+    std::vector<int> pending(gates.size());
+    for (size_t i = 0; i < gates.size(); i++) pending[i] = (int)i;
+    std::vector<char> in(n), blocked(n);
+    while (!pending.empty()) {
+        std::fill(in.begin(), in.end(), 0);
+        int size = 0;
+        for (int q : fixed) if (!in[q]) { in[q] = 1; size++; }
+        int best = count_runnable(gates, pending, in, blocked, nullptr);
+        while (size < budget) {
+            int best_q = -1, best_c = best;
+            for (int q = 0; q < n; q++) {
+                if (in[q]) continue;
+                in[q] = 1;
+                int c = count_runnable(gates, pending, in, blocked, nullptr);
+                in[q] = 0;
+                if (c > best_c) { best_c = c; best_q = q; }
+            }
+            if (best_q >= 0) {
+                in[best_q] = 1; size++; best = best_c;
+                continue;
+            }
+            // No single qubit helps: add every missing qubit of the first
+            // blocked gate if they fit (a two-qubit gate needs both at once).
+            bool added = false;
+            for (int gi : pending) {
+                const sq_gate &g = gates[gi];
+                int missing = 0;
+                for (int i = 0; i < g.nq; i++) missing += !in[g.q[i]];
+                if (missing == 0) continue;
+                if (size + missing > budget) break;
+                for (int i = 0; i < g.nq; i++) if (!in[g.q[i]]) { in[g.q[i]] = 1; size++; }
+                best = count_runnable(gates, pending, in, blocked, nullptr);
+                added = true;
+                break;
+            }
+            if (!added) break;
+        }
+        PlanPass p;
+        count_runnable(gates, pending, in, blocked, &p.gates);
+        if (p.gates.empty()) {
+            // The first pending gate alone exceeds the budget with the fixed set:
+            // schedule it by itself (callers size budget >= |fixed| + 4).
+            p.gates.push_back(pending.front());
+            for (int q : gate_qubits(gates[pending.front()])) in[q] = 1;
+        }
+        for (int q = 0; q < n; q++) if (in[q]) p.qubits.push_back(q);
+        std::vector<int> rest;
+        size_t k = 0;
+        for (int gi : pending) {
+            if (k < p.gates.size() && p.gates[k] == gi) { k++; continue; }
+            rest.push_back(gi);
+        }
+        pending.swap(rest);
+        passes.push_back(std::move(p));
+    }
+    return passes;
+}
+
+std::string plan_subcircuit_text(const std::vector<sq_gate> &gates, int n, int chunk, bool restore) {
+    std::vector<int> pos(n), at(n);
+    for (int q = 0; q < n; q++) pos[q] = at[q] = q;
+    std::string out;
+    auto emit_swaps = [&](std::vector<std::pair<int, int>> sw) {
+        // sw: (chunk slot, high slot) physical pairs; pair them into VSWAP_2_2
+        // where possible (a < b < chunk <= c < d), else VSWAP_1_1.
+        std::vector<int> lo, hi;
+        for (auto &p : sw) { lo.push_back(p.first); hi.push_back(p.second); }
+        std::sort(lo.begin(), lo.end());
+        std::sort(hi.begin(), hi.end());
+        size_t i = 0;
+        for (; i + 1 < lo.size(); i += 2) {
+            out += "1\nVSWAP_2_2 " + std::to_string(lo[i]) + " " + std::to_string(lo[i + 1]) + " " +
+                   std::to_string(hi[i]) + " " + std::to_string(hi[i + 1]) + "\n";
+            for (int j = 0; j < 2; j++) {
+                int a = lo[i + j], b = hi[i + j];
+                std::swap(at[a], at[b]);
+                pos[at[a]] = a; pos[at[b]] = b;
+            }
+        }
+        for (; i < lo.size(); i++) {
+            out += "1\nVSWAP_1_1 " + std::to_string(lo[i]) + " " + std::to_string(hi[i]) + "\n";
+            int a = lo[i], b = hi[i];
+            std::swap(at[a], at[b]);
+            pos[at[a]] = a; pos[at[b]] = b;
+        }
+    };
+    auto passes = plan_passes(gates, n, chunk, {});
+    for (auto &p : passes) {
+        std::vector<char> need(n, 0);
+        for (int q : p.qubits) need[q] = 1;
+        std::vector<int> free_slots;
+        for (int s = 0; s < chunk; s++) if (!need[at[s]]) free_slots.push_back(s);
+        std::vector<std::pair<int, int>> sw;
+        size_t f = 0;
+        for (int q : p.qubits)
+            if (pos[q] >= chunk) sw.push_back({free_slots[f++], pos[q]});
+        if (!sw.empty()) emit_swaps(sw);
+        out += std::to_string(p.gates.size()) + "\n";
+        for (int gi : p.gates) {
+            sq_gate g = gates[gi];
+            for (int i = 0; i < g.nq; i++) g.q[i] = pos[g.q[i]];
+            if (g.nq == 2 && g.q[0] > g.q[1]) {
+                std::swap(g.q[0], g.q[1]);
+                if (g.kind == SQ_GATE_U2) {
+                    // reorder the 4x4 basis |first second> -> |second first>
+                    static const int perm[4] = {0, 2, 1, 3};
+                    double m[32];
+                    for (int r = 0; r < 4; r++)
+                        for (int c = 0; c < 4; c++)
+                            for (int z = 0; z < 2; z++) m[2 * (4 * r + c) + z] = g.mat[2 * (4 * perm[r] + perm[c]) + z];
+                    std::memcpy(g.mat, m, sizeof m);
+                }
+            }
+            out += gate_to_line(g) + "\n";
+        }
+    }
+    if (restore) {
+        // Return every logical qubit to its own index with pairwise swaps.
+        for (int p = 0; p < n; p++) {
+            while (at[p] != p) {
+                int q = at[p];          // logical qubit sitting at p belongs at q
+                int a = std::min(p, q), b = std::max(p, q);
+                out += "1\nVSWAP_1_1 " + std::to_string(a) + " " + std::to_string(b) + "\n";
+                std::swap(at[a], at[b]);
+                pos[at[a]] = a; pos[at[b]] = b;
+            }
+        }
+    }
+    return out;
+}
+
+}  // namespace sq
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" int sq_plan(const char *circuit_text, int32_t n, int32_t mode, int32_t param, char *buf,
+                       uint64_t cap, uint64_t *len) {
+    if (!circuit_text || n < 1 || n > 64) return SQ_ERR_ARG;
+    uint64_t ng = 0, nb = 0;
+    int32_t line = 0;
+    int rc = sq_parse_circuit(circuit_text, n, 0, nullptr, 0, &ng, nullptr, 0, &nb, &line);
+    if (rc != SQ_OK && rc != SQ_ERR_ARG) return rc;
+    std::vector<sq_gate> gates(std::max<uint64_t>(ng, 1));
+    rc = sq_parse_circuit(circuit_text, n, 0, gates.data(), ng, &ng, nullptr, 0, &nb, &line);
+    if (rc != SQ_OK) return rc;
+    gates.resize(ng);
+    std::string out;
+    if (mode == 0 || mode == 3) {
+        // finder role: reference subcircuit text for chunk_qbit = param
+        // (mode 3 also restores the qubit order at the end)
+        if (param < 4 || param > n) return SQ_ERR_ARG;
+        out = sq::plan_subcircuit_text(gates, n, param, mode == 3);
+    } else if (mode == 1) {
+        // fused blocks: every block touches at most param qubits (no relabelling)
+        if (param < 1 || param > 6) return SQ_ERR_ARG;
+        for (auto &p : sq::plan_passes(gates, n, param, {})) {
+            out += std::to_string(p.gates.size()) + "\n";
+            for (int gi : p.gates) out += sq::gate_to_line(gates[gi]) + "\n";
+        }
+    } else {
+        return SQ_ERR_ARG;
+    }
+    if (len) *len = out.size() + 1;
+    if (!buf || cap < out.size() + 1) return SQ_ERR_ARG;
+    std::memcpy(buf, out.c_str(), out.size() + 1);
+    return SQ_OK;
+}

@@ -1,0 +1,40 @@
+// planner.hpp -- groups a gate sequence into passes whose gates all act inside
+// a small qubit set.  This is the role of the reference's subcircuit finder
+// (subcircuitFinder/finder.cpp:629-877): it greedily picks the chunk set that
+// lets the most pending gates run (findChunkbyMaxNodePrinciple, 651-698) and
+// pops them (popNodes, 629-649).  Here the same greedy rule is parameterised:
+//   * budget: qubits per pass (chunk_qbit, LDS tile width, or fused k),
+//   * fixed:  qubits every pass must contain (the low index bits, so a pass's
+//             tile is read from HBM in whole contiguous runs).
+// Dependencies are per qubit: a gate may join a pass only if every earlier
+// gate sharing a qubit with it has already been scheduled.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "subquokka_hip.h"
+
+namespace sq {
+
+struct PlanPass {
+    std::vector<int> qubits;     // sorted; |qubits| <= budget
+    std::vector<int> gates;      // indices into the input, in execution order
+};
+
+std::vector<int> gate_qubits(const sq_gate &g);
+
+// Greedy pass partition.  fixed must be a subset of [0, n) of size <= budget.
+std::vector<PlanPass> plan_passes(const std::vector<sq_gate> &gates, int n, int budget,
+                                  const std::vector<int> &fixed);
+
+// Reference subcircuit text (finder.cpp:69-333 format) for a chunk of
+// chunk_qbit low qubits: gate blocks with relabelled targets separated by
+// VSWAP_1_1 / VSWAP_2_2 passes.  restore=true appends swaps that return every
+// logical qubit to its own position, which the reference finder leaves out
+// (finder.cpp:304-330 is commented out).
+std::string plan_subcircuit_text(const std::vector<sq_gate> &gates, int n, int chunk_qbit, bool restore);
+
+// Gate line in the MEM grammar with %.17g reals (parses back to the same doubles).
+std::string gate_to_line(const sq_gate &g);
+
+}  // namespace sq

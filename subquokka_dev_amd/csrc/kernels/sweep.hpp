@@ -23,7 +23,7 @@ template <typename T> struct DenseArgs {
     int lmask[2];       // lane masks 1 << (t - VB) of lane targets, ascending
 };
 
-template <typename T, int KV, int KL, int KH>
+template <typename T, int KV, int KL, int KH, int U>
 __global__ void __launch_bounds__(kBlock) k_dense(cx<T> *__restrict__ s, const DenseArgs<T> a) {
     constexpr int VB = Traits<T>::VB;
     constexpr int S = VB + 6;
@@ -39,59 +39,66 @@ __global__ void __launch_bounds__(kBlock) k_dense(cx<T> *__restrict__ s, const D
 #pragma unroll
     for (int j = 0; j < KL; j++) c |= ((lane & a.lmask[j]) ? 1 : 0) << j;
 
-    for (uint64_t tile = wave; tile < a.n_tiles; tile += n_waves) {
-        uint64_t base = tile << S;
+    // U consecutive wave tiles per iteration: all loads issue before any use
+    for (uint64_t t0 = wave * U; t0 < a.n_tiles; t0 += n_waves * U) {
+        uint64_t base[U];
+        Pack<T> reg[U][R];
 #pragma unroll
-        for (int j = 0; j < KH; j++) base = insert_zero(base, a.hpos[j]);
-        base += (uint64_t)lane << VB;
-        Pack<T> reg[R];
+        for (int u = 0; u < U; u++) {
+            base[u] = (t0 + u) << S;
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            uint64_t off = base;
+            for (int j = 0; j < KH; j++) base[u] = insert_zero(base[u], a.hpos[j]);
+            base[u] += (uint64_t)lane << VB;
+            if (t0 + u >= a.n_tiles) continue;   // uniform per wave
 #pragma unroll
-            for (int j = 0; j < KH; j++) if ((r >> j) & 1) off += 1ULL << a.hpos[j];
-            reg[r] = ld(s + off);
-        }
-        // sh[d][r]: register r of the lane that differs from this one by the lane
-        // targets selected in d.
-        Pack<T> sh[NL][R];
+            for (int r = 0; r < R; r++) {
+                uint64_t off = base[u];
 #pragma unroll
-        for (int d = 0; d < NL; d++) {
-            int mask = 0;
-#pragma unroll
-            for (int j = 0; j < KL; j++) if ((d >> j) & 1) mask |= a.lmask[j];
-#pragma unroll
-            for (int r = 0; r < R; r++) sh[d][r] = d == 0 ? reg[r] : shfl_xor_pack(reg[r], mask);
-        }
-        Pack<T> out[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-#pragma unroll
-            for (int e = 0; e < E; e++) {
-                cx<T> res = {0, 0};
-#pragma unroll
-                for (int cc = 0; cc < NL; cc++) {   // candidate lane bits, selected below
-                    const int row = (KV ? (e & 1) : 0) | (cc << KV) | (r << (KV + KL));
-                    cx<T> acc = {0, 0};
-#pragma unroll
-                    for (int b = 0; b < D; b++) {
-                        const int bv = KV ? (b & 1) : 0;
-                        const int bc = (b >> KV) & (NL - 1);
-                        const int br = b >> (KV + KL);
-                        const int ee = KV ? ((e & ~1) | bv) : e;
-                        acc = cfma(acc, a.m[row * D + b], sh[cc ^ bc][br].e[ee]);
-                    }
-                    if (cc == c) res = acc;
-                }
-                out[r].e[e] = res;
+                for (int j = 0; j < KH; j++) if ((r >> j) & 1) off += 1ULL << a.hpos[j];
+                reg[u][r] = ld(s + off);
             }
         }
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            uint64_t off = base;
+        for (int u = 0; u < U; u++) {
+            if (t0 + u >= a.n_tiles) continue;
+            // sh[d][r]: register r of the lane that differs from this one by the
+            // lane targets selected in d.
+            Pack<T> sh[NL][R];
 #pragma unroll
-            for (int j = 0; j < KH; j++) if ((r >> j) & 1) off += 1ULL << a.hpos[j];
-            st(s + off, out[r]);
+            for (int d = 0; d < NL; d++) {
+                int mask = 0;
+#pragma unroll
+                for (int j = 0; j < KL; j++) if ((d >> j) & 1) mask |= a.lmask[j];
+#pragma unroll
+                for (int r = 0; r < R; r++) sh[d][r] = d == 0 ? reg[u][r] : shfl_xor_pack(reg[u][r], mask);
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                Pack<T> out;
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    cx<T> res = {0, 0};
+#pragma unroll
+                    for (int cc = 0; cc < NL; cc++) {   // candidate lane bits, selected below
+                        const int row = (KV ? (e & 1) : 0) | (cc << KV) | (r << (KV + KL));
+                        cx<T> acc = {0, 0};
+#pragma unroll
+                        for (int b = 0; b < D; b++) {
+                            const int bv = KV ? (b & 1) : 0;
+                            const int bc = (b >> KV) & (NL - 1);
+                            const int br = b >> (KV + KL);
+                            const int ee = KV ? ((e & ~1) | bv) : e;
+                            acc = cfma(acc, a.m[row * D + b], sh[cc ^ bc][br].e[ee]);
+                        }
+                        if (cc == c) res = acc;
+                    }
+                    out.e[e] = res;
+                }
+                uint64_t off = base[u];
+#pragma unroll
+                for (int j = 0; j < KH; j++) if ((r >> j) & 1) off += 1ULL << a.hpos[j];
+                st(s + off, out);
+            }
         }
     }
 }
@@ -151,19 +158,30 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_diag(cx<T> *__restrict__ s, const DiagArgs<T> a) {
     constexpr int VB = Traits<T>::VB;
     constexpr int E = 1 << VB;
-    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < a.n_vec; v += stride) {
-        uint64_t i = v << VB;
-        for (int j = 0; j < a.nfixed; j++) i = insert_one(i, a.fixed[j]);
-        Pack<T> p = ld(s + i);
+    constexpr int U = 2;   // vectors per thread per iteration, one grid apart
+    const uint64_t span = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < a.n_vec; v0 += span * U) {
+        uint64_t i[U];
+        Pack<T> p[U];
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-            const uint64_t idx = i + e;
-            int l = 0;
-            for (int j = 0; j < a.k; j++) l |= (int)((idx >> a.t[j]) & 1) << j;
-            p.e[e] = cmul(p.e[e], a.d[l]);
+        for (int u = 0; u < U; u++) {
+            const uint64_t v = v0 + u * span;
+            i[u] = v << VB;
+            for (int j = 0; j < a.nfixed; j++) i[u] = insert_one(i[u], a.fixed[j]);
+            if (v < a.n_vec) p[u] = ld(s + i[u]);
         }
-        st(s + i, p);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (v0 + u * span >= a.n_vec) continue;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const uint64_t idx = i[u] + e;
+                int l = 0;
+                for (int j = 0; j < a.k; j++) l |= (int)((idx >> a.t[j]) & 1) << j;
+                p[u].e[e] = cmul(p[u].e[e], a.d[l]);
+            }
+            st(s + i[u], p[u]);
+        }
     }
 }
 

@@ -1,10 +1,14 @@
 // sq_blocks.hip -- subcircuit (blocked) sweep: MEM_Runner::run(subcircuits)
 // (MEM_Runner.cpp:136-185).
 //
-// A subcircuit file alternates VSWAP blocks (qubit-swap passes) and gate
-// blocks whose targets all lie below chunk_qbit.  The reference applies a
-// gate block tile by tile: for every 2^chunk-amplitude tile, all gates of the
-// block in order (MEM_Runner.cpp:173-182), so the tile stays in cache.
+// A subcircuit file alternates VSWAP blocks (qubit-swap passes, applied as
+// permutation sweeps) and gate blocks.  The reference applies a gate block
+// tile by tile (MEM_Runner.cpp:173-182); here a gate block becomes one LDS
+// tile pass (kernels/tile.hpp) over a tile qubit set Q that contains the
+// block's targets and the lowest index bits.  A block too wide for one tile
+// is split by the planner into several tile passes; the result is the same
+// because the gates are applied in an order that respects every per-qubit
+// dependency.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,8 +17,209 @@
 
 #include "subquokka_hip.h"
 #include "engine_internal.hpp"
+#include "host/planner.hpp"
+#include "kernels/tile.hpp"
 
 using sq::LoweredGate;
+using sq::OpClass;
+using namespace sqk;
+
+namespace {
+
+constexpr int kTileMinK = 10;   // 4 amplitudes per thread with 256 threads
+
+// The LDS tile pass is switched off: its first GPU runs (the 21-qubit QFT
+// subcircuit fixture, chunk_qbit 12) ended in an illegal memory access whose
+// cause is not yet shown.  Gate blocks run as per-gate sweeps, which compute
+// the same map (every gate of a block is tile-local).
+#ifndef SQ_TILE_PASSES
+#define SQ_TILE_PASSES 0
+#endif
+constexpr bool kTilePassesEnabled = SQ_TILE_PASSES != 0;
+constexpr int kTilePrefK = 10;  // 16 KiB (complex128) per workgroup: high occupancy
+
+// Widest tile whose 2^K amplitudes fit the device's per-workgroup LDS limit.
+int tile_max_k(const sq_engine *e) {
+    int k = e->dtype == SQ_C128 ? 12 : 13;
+    while (k > kTileMinK && ((size_t)1 << k) * e->amp_bytes() > e->lds_per_block) k--;
+    return k;
+}
+int low_fixed(int dtype) { return dtype == SQ_C128 ? 6 : 7; }      // one wave's 1 KiB run
+
+struct TilePass {
+    bool sweep = false;                 // true: apply gates one by one (VSWAP blocks, tiny states)
+    std::vector<int> gates;             // indices into the lowered list
+    std::vector<int> q;                 // tile qubits ascending (tile passes)
+    int op_begin = 0, n_ops = 0;        // slice of the op array
+};
+
+template <typename T>
+int launch_tile(sq_engine *e, int K, const TileArgs<T> &a, unsigned grid) {
+    cx<T> *s = reinterpret_cast<cx<T> *>(e->d_state);
+    const size_t lds = ((size_t)1 << K) * sizeof(cx<T>);
+    if (lds > e->lds_per_block) return e->fail(SQ_ERR_ARG, "internal: tile exceeds the LDS limit");
+    switch (K) {
+    case 10: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile<T, 10, 256>), grid, 256, lds, e->stream, s, a); break;
+    case 11: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile<T, 11, 256>), grid, 256, lds, e->stream, s, a); break;
+    case 12: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile<T, 12, 256>), grid, 256, lds, e->stream, s, a); break;
+    case 13:
+        if constexpr (sizeof(T) == 4) { hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile<T, 13, 256>), grid, 256, lds, e->stream, s, a); break; }
+        return e->fail(SQ_ERR_ARG, "internal: tile too wide for complex128");
+    default:
+        return e->fail(SQ_ERR_ARG, "internal: bad tile width");
+    }
+    return e->check_launch("k_tile");
+}
+
+// Appends one gate as a tile op (local bit positions) and its coefficients.
+template <typename T>
+void encode_op(const LoweredGate &g, const std::vector<int> &q, std::vector<TileOp> &ops, std::vector<cx<T>> &coef) {
+    auto local = [&](int t) { return (uint8_t)(std::lower_bound(q.begin(), q.end(), t) - q.begin()); };
+    auto push_coef = [&](int cnt) {
+        for (int i = 0; i < cnt; i++) coef.push_back({(T)g.m[i].real(), (T)g.m[i].imag()});
+    };
+    TileOp o{};
+    o.coef = (uint32_t)coef.size();
+    if (g.cls == OpClass::Perm) {
+        for (int p = 0; p < g.n_pairs; p++) {
+            TileOp w{};
+            int a = local(g.t[2 * p]), b = local(g.t[2 * p + 1]);
+            w.kind = TK_SWAP; w.t0 = (uint8_t)std::min(a, b); w.t1 = (uint8_t)std::max(a, b);
+            w.coef = (uint32_t)coef.size();
+            ops.push_back(w);
+        }
+        return;
+    }
+    o.t0 = local(g.t[0]);
+    o.t1 = g.k > 1 ? local(g.t[1]) : 0;
+    if (g.cls == OpClass::Dense) {
+        o.kind = g.k == 1 ? TK_DENSE1 : TK_DENSE2;
+        push_coef(g.k == 1 ? 4 : 16);
+    } else {
+        o.kind = g.k == 1 ? TK_DIAG1 : TK_DIAG2;
+        push_coef(g.k == 1 ? 2 : 4);
+    }
+    ops.push_back(o);
+}
+
+template <typename T>
+int run_blocks(sq_engine *e, const sq_gate *gates, const std::vector<LoweredGate> &low,
+               const uint32_t *block_sizes, uint64_t n_blocks) {
+    const int n = e->n;
+    const int kmax = std::min(tile_max_k(e), n);
+    std::vector<int> fixed;
+    for (int i = 0; i < std::min(low_fixed(e->dtype), n); i++) fixed.push_back(i);
+
+    // 1. passes
+    std::vector<TilePass> passes;
+    size_t first = 0;
+    for (uint64_t b = 0; b < n_blocks; b++) {
+        const size_t cnt = block_sizes[b];
+        if (cnt == 0) continue;
+        const int kind0 = gates[first].kind;
+        const bool vswap = kind0 == SQ_GATE_VSWAP_1_1 || kind0 == SQ_GATE_VSWAP_2_2;
+        if (vswap || n < kTileMinK || !kTilePassesEnabled) {
+            TilePass p;
+            p.sweep = true;
+            for (size_t i = 0; i < cnt; i++) p.gates.push_back((int)(first + i));
+            passes.push_back(std::move(p));
+        } else {
+            std::vector<char> need(n, 0);
+            for (int f : fixed) need[f] = 1;
+            for (size_t i = 0; i < cnt; i++)
+                for (int j = 0; j < gates[first + i].nq; j++) need[gates[first + i].q[j]] = 1;
+            const int req = (int)std::count(need.begin(), need.end(), 1);
+            std::vector<std::vector<int>> groups;   // (gate indices, required qubits)
+            std::vector<std::vector<int>> qsets;
+            if (req <= kmax) {
+                std::vector<int> gi, qs;
+                for (size_t i = 0; i < cnt; i++) gi.push_back((int)(first + i));
+                for (int q = 0; q < n; q++) if (need[q]) qs.push_back(q);
+                groups.push_back(gi);
+                qsets.push_back(qs);
+            } else {
+                std::vector<sq_gate> blk(gates + first, gates + first + cnt);
+                for (auto &pp : sq::plan_passes(blk, n, kmax, fixed)) {
+                    std::vector<int> gi;
+                    for (int x : pp.gates) gi.push_back((int)first + x);
+                    groups.push_back(gi);
+                    qsets.push_back(pp.qubits);
+                }
+            }
+            for (size_t gidx = 0; gidx < groups.size(); gidx++) {
+                TilePass p;
+                p.gates = groups[gidx];
+                std::vector<char> in(n, 0);
+                for (int q : qsets[gidx]) in[q] = 1;
+                for (int f : fixed) in[f] = 1;
+                int size = (int)std::count(in.begin(), in.end(), 1);
+                const int want = std::min(std::max(size, kTilePrefK), kmax);
+                for (int q = 0; q < n && size < want; q++) if (!in[q]) { in[q] = 1; size++; }
+                for (int q = 0; q < n; q++) if (in[q]) p.q.push_back(q);
+                passes.push_back(std::move(p));
+            }
+        }
+        first += cnt;
+    }
+
+    // 2. tile programs, uploaded once
+    std::vector<TileOp> ops;
+    std::vector<cx<T>> coef;
+    for (auto &p : passes) {
+        if (p.sweep) continue;
+        p.op_begin = (int)ops.size();
+        for (int gi : p.gates) encode_op<T>(low[gi], p.q, ops, coef);
+        p.n_ops = (int)ops.size() - p.op_begin;
+    }
+    const size_t ops_bytes = ((ops.size() * sizeof(TileOp) + 255) / 256) * 256;
+    const size_t need_bytes = ops_bytes + coef.size() * sizeof(cx<T>) + 256;
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return e->fail(SQ_ERR_HIP, "stream sync");
+    if (need_bytes > e->blockprog_cap) {
+        if (e->d_blockprog) hipFree(e->d_blockprog);
+        e->d_blockprog = nullptr;
+        e->blockprog_cap = 0;
+        hipError_t he = hipMalloc(&e->d_blockprog, need_bytes * 2);
+        if (he != hipSuccess) return e->hip_fail(he, "hipMalloc(tile program)");
+        e->blockprog_cap = need_bytes * 2;
+    }
+    char *dprog = static_cast<char *>(e->d_blockprog);
+    if (!ops.empty()) {
+        hipError_t he = hipMemcpy(dprog, ops.data(), ops.size() * sizeof(TileOp), hipMemcpyHostToDevice);
+        if (he == hipSuccess && !coef.empty())
+            he = hipMemcpy(dprog + ops_bytes, coef.data(), coef.size() * sizeof(cx<T>), hipMemcpyHostToDevice);
+        if (he != hipSuccess) return e->hip_fail(he, "hipMemcpy(tile program)");
+    }
+    const TileOp *dops = reinterpret_cast<const TileOp *>(dprog);
+    const cx<T> *dcoef = reinterpret_cast<const cx<T> *>(dprog + ops_bytes);
+
+    // 3. launches in order
+    for (auto &p : passes) {
+        if (p.sweep) {
+            for (int gi : p.gates) {
+                int rc = e->apply_lowered(low[gi]);
+                if (rc) return rc;
+            }
+            continue;
+        }
+        TileArgs<T> a{};
+        a.ops = dops + p.op_begin;
+        a.coef = dcoef;
+        a.n_ops = p.n_ops;
+        const int K = (int)p.q.size();
+        for (int i = 0; i < K; i++) a.q[i] = p.q[i];
+        a.n_tiles = 1ULL << (n - K);
+        char nm[48];
+        std::snprintf(nm, sizeof nm, "tile%d_%s", K, e->dtype == SQ_C128 ? "c128" : "c64");
+        int slot = e->stat_begin(nm, 2.0 * e->amp_bytes() * std::ldexp(1.0, n));
+        int rc = launch_tile<T>(e, K, a, (unsigned)a.n_tiles);
+        e->stat_end(slot);
+        if (rc) return rc;
+        e->n_tile_passes++;
+    }
+    return SQ_OK;
+}
+
+}  // namespace
 
 extern "C" int sq_apply_blocks(sq_engine *e, const sq_gate *gates, const uint32_t *block_sizes,
                                uint64_t n_blocks) {
@@ -27,13 +232,8 @@ extern "C" int sq_apply_blocks(sq_engine *e, const sq_gate *gates, const uint32_
         int rc = e->lower(gates[i], low[i]);
         if (rc) { e->err += " (gate " + std::to_string(i) + ")"; return rc; }
     }
-    // Gate blocks: every gate is tile-local, so applying the block's gates in
-    // order over the whole state is the same map as applying them tile by tile.
-    for (uint64_t i = 0; i < total; i++) {
-        int rc = e->apply_lowered(low[i]);
-        if (rc) return rc;
-    }
-    return SQ_OK;
+    return e->dtype == SQ_C128 ? run_blocks<double>(e, gates, low, block_sizes, n_blocks)
+                               : run_blocks<float>(e, gates, low, block_sizes, n_blocks);
 }
 
 void sq_engine::free_extras() {
@@ -55,7 +255,4 @@ extern "C" int sq_hea_energy(sq_engine *e, int32_t, const double *, double *) {
 }
 extern "C" int sq_adjoint_grad_hea(sq_engine *e, int32_t, const double *, int32_t, double *, double *) {
     return e ? e->fail(SQ_ERR_UNSUPPORTED, "adjoint not built yet") : SQ_ERR_ARG;
-}
-extern "C" int sq_plan(const char *, int32_t, int32_t, int32_t, char *, uint64_t, uint64_t *) {
-    return SQ_ERR_UNSUPPORTED;
 }

@@ -107,3 +107,44 @@ def test_perm_map_touches_only_moving_amplitudes(vb):
             return any(((i >> p) & 1) != ((i >> r) & 1) for p, r in pairs)
         want = [i for i in range(1 << n) if moving(i)]
         assert sorted(a.tolist()) == want
+
+
+def tile_addresses(n, q, bs=256):
+    """Global offsets touched by k_tile (kernels/tile.hpp) for tile set q."""
+    k = len(q)
+    lb = bs.bit_length() - 1
+    m_per = (1 << k) // bs
+    tiles = np.arange(1 << (n - k), dtype=np.int64)
+    base = tiles.copy()
+    for b in q:
+        base = insert_zero(base, b)
+    out = []
+    for tid in range(bs):
+        off_lo = sum(1 << q[i] for i in range(lb) if (tid >> i) & 1)
+        for m in range(m_per):
+            off_hi = sum(1 << q[i] for i in range(lb, k) if ((m << lb) >> i) & 1)
+            out.append(base + off_lo + off_hi)
+    return np.concatenate(out)
+
+
+@pytest.mark.parametrize("q", [list(range(10)), [0, 1, 2, 3, 4, 5, 7, 9, 11, 13], list(range(6)) + [8, 10, 12, 13, 14, 15]])
+def test_tile_map_is_bijection(q):
+    n = 16
+    a = tile_addresses(n, q)
+    assert a.min() >= 0 and a.max() < (1 << n)
+    assert len(np.unique(a)) == 1 << n
+
+
+def test_tile_lds_indices_in_range():
+    for k in (10, 11, 12, 13):
+        d = 1 << k
+        for b in range(k):
+            p = np.arange(d // 2)
+            i0 = insert_zero(p, b)
+            assert i0.max() | (1 << b) < d and len(np.unique(np.concatenate([i0, i0 | (1 << b)]))) == d
+        for b0 in range(k):
+            for b1 in range(b0 + 1, k):
+                p = np.arange(d // 4)
+                i = insert_zero(insert_zero(p, b0), b1)
+                allq = np.concatenate([i, i | (1 << b0), i | (1 << b1), i | (1 << b0) | (1 << b1)])
+                assert allq.max() < d and len(np.unique(allq)) == d
