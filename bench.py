@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c64", action="store_true")
     ap.add_argument("--cpu-sample-gates", type=int, default=45)
+    ap.add_argument("--no-fused", action="store_true")
+    ap.add_argument("--fused-k", type=int, default=5)
     return ap.parse_args()
 
 
@@ -151,6 +153,57 @@ def main():
                       "layers": layers, "gates": ng, "state": "complex128 in HBM",
                       "mode": "plain gate-by-gate (MEM_Runner::run(vector<Gate*>&) form)"},
            "hbm_gbs": round(hbm_gbs, 1), "roofline": roofline, "kernels_c128": kernels}
+
+    if not args.no_fused:
+        # Planner (host, once per circuit, outside the timed region -- the
+        # reference runs its finder offline too), then per step: FP64
+        # composition of every block on the GPU + MFMA application.
+        t0 = time.perf_counter()
+        plan = sq.plan(text, n, 1, args.fused_k)
+        plan_s = time.perf_counter() - t0
+        fg, fb, fng, fnb = sq.parse_circuit(plan, n, is_subcircuit=True)
+        print(f"[bench] fused plan: {fnb} blocks of <= {args.fused_k} qubits ({plan_s:.2f} s)", file=sys.stderr, flush=True)
+        fused = {}
+        for dtype in ("c64", "c128"):
+            eng = sq.Engine(n, dtype=dtype)
+            for _ in range(args.warmup):
+                eng.init_zero()
+                eng.run_fused(fg, fb, fnb)
+            eng.sync()
+            eng.stats_reset()
+            eng.stats_enable(True)
+            times = []
+            for _ in range(args.steps):
+                eng.init_zero()
+                eng.sync()
+                t0 = time.perf_counter()
+                eng.run_fused(fg, fb, fnb)
+                eng.sync()
+                times.append(time.perf_counter() - t0)
+                print(f"[bench] fused {dtype} step {len(times)}: {times[-1]*1e3:.1f} ms", file=sys.stderr, flush=True)
+            st = eng.stats()
+            eng.stats_enable(False)
+            k = st.get(f"fused5_{dtype}", {"launches": 1, "total_ms": float("nan"), "alg_bytes": 0})
+            avg_ms = k["total_ms"] / k["launches"]
+            bpl = k["alg_bytes"] / k["launches"]
+            flops = 8.0 * 32 * 2 ** n     # complex 32x32 per 32-amplitude column
+            peak_tf = 157.3 if dtype == "c64" else 78.6
+            ov = eng128.inner(eng)
+            infid = 1.0 - abs(ov) ** 2 / (eng128.norm2() * eng.norm2())
+            tt = sum(times)
+            fused[dtype] = {
+                "gates_s": round(args.steps * ng / tt, 2), "ms_per_step": round(tt / args.steps * 1e3, 2),
+                "blocks": fnb, "speedup_vs_plain_c128": round((total_t / args.steps) / (tt / args.steps), 2),
+                "infidelity_vs_plain_c128": infid, "max_abs_err_vs_plain_c128": eng128.max_abs_diff(eng),
+                "kernel": {"avg_launch_ms": round(avg_ms, 4), "hbm_gbs": round(bpl / (avg_ms * 1e-3) / 1e9, 1),
+                           "hbm_frac": round(bpl / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "mfma_tflops": round(flops / (avg_ms * 1e-3) / 1e12, 1),
+                           "mfma_frac": round(flops / (avg_ms * 1e-3) / 1e12 / peak_tf, 4),
+                           "mfma_peak_tflops": peak_tf},
+                "compose_ms_per_step": round(st.get("compose_f64", {"total_ms": 0})["total_ms"] / args.steps, 3),
+                "plan_s": round(plan_s, 3)}
+            eng.close()
+        out["fused"] = fused
 
     if not args.no_c64:
         eng64, t64, st64 = run_steps("c64", args.steps, args.warmup, False)

@@ -119,8 +119,14 @@ int sq_apply_gates(sq_engine *e, const sq_gate *gates, uint64_t n);
  * qubit-swap passes, the others are applied tile by tile in LDS. */
 int sq_apply_blocks(sq_engine *e, const sq_gate *gates, const uint32_t *block_sizes,
                     uint64_t n_blocks);
-/* Dense fused blocks on MFMA (no reference counterpart, see sq_fused). */
+/* Dense fused blocks on MFMA (no reference counterpart, see sq_fused).
+ * Matrices are supplied by the caller in FP64. */
 int sq_apply_fused(sq_engine *e, const sq_fused *blocks, uint64_t n);
+/* Fused blocks from gates: block i is block_sizes[i] consecutive gates acting on
+ * at most five qubits (the planner's mode 1 output).  Each block is composed in
+ * FP64 on the GPU, then applied on MFMA (FP32 accumulation for SQ_C64, FP64 for
+ * SQ_C128).  The result equals applying the gates in order. */
+int sq_run_fused(sq_engine *e, const sq_gate *gates, const uint32_t *block_sizes, uint64_t n_blocks);
 /* Composes gates (all targets inside q[0..k-1]) into one 2^k x 2^k FP64 matrix on
  * the GPU; u_out receives 2*4^k doubles. */
 int sq_compose_block(sq_engine *e, const sq_gate *gates, uint64_t n, int32_t k,

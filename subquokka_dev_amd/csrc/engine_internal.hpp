@@ -37,9 +37,29 @@ struct sq_engine {
     void *d_fused = nullptr;        // fused-block matrices
     size_t fused_cap = 0;
     void *d_aux = nullptr;          // adjoint: second state (lambda)
-    void *d_ckpt = nullptr;         // adjoint: checkpoints
+    size_t aux_cap = 0;
+    void *d_ckpt = nullptr;         // adjoint: FP64 checkpoints
     size_t ckpt_cap = 0;
+    void *d_gradpart = nullptr;     // adjoint: per-parameter block partials
+    size_t gradpart_cap = 0;
     uint64_t n_tile_passes = 0;     // tile passes launched (diagnostics)
+
+    // out-of-core runner (resident_qbit > 0): state in pinned host memory
+    int res_qbit = 0;
+    void *h_state = nullptr;
+    void *d_group[2] = {nullptr, nullptr};
+    hipEvent_t ev_loaded[2] = {nullptr, nullptr}, ev_computed[2] = {nullptr, nullptr},
+               ev_stored[2] = {nullptr, nullptr}, ev_pass = nullptr;
+    hipStream_t s_h2d = nullptr, s_d2h = nullptr;
+    uint64_t n_stream_passes = 0;
+    bool out_of_core() const { return res_qbit > 0; }
+    int stream_create();
+    void stream_destroy();
+    int stream_init_zero();
+    int stream_set_state(const void *host, int32_t host_dtype);
+    int stream_get_state(void *host, int32_t host_dtype);
+    int stream_norm_zz(double *v);
+    int stream_apply(const std::vector<sq_gate> &gates, const std::vector<sq::LoweredGate> &low);
 
     uint64_t n_amps() const { return 1ULL << n; }
     size_t amp_bytes() const { return dtype == SQ_C128 ? 16 : 8; }

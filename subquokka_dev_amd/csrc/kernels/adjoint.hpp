@@ -1,0 +1,95 @@
+// adjoint.hpp -- kernels of the adjoint (reverse-sweep) gradient.
+//
+// For a rotation G(theta) = exp(-i theta P / 2), P in {Y, Z} (RY, RZ with the
+// reference's matrices, circuit.cpp:2736-2737, 2764-2765), and energy
+// E = <psi|H|psi>, the reverse sweep holds psi_k (state after gate k) and
+// lambda_k = G_{k+1}^dag ... G_L^dag H psi_L and computes
+//     dE/dtheta_k = 2 Re <lambda_k| dG_k/dtheta |psi_{k-1}> = Im <lambda_k| P |psi_k>,
+// then un-applies G_k to both vectors.  One kernel does all three in a single
+// pass over psi and lambda; the inner product is accumulated in FP64.
+#pragma once
+#include "common.hpp"
+#include "reduce.hpp"
+
+namespace sqk {
+
+struct AdjRotArgs {
+    uint64_t n_pairs;
+    int q;
+    int axis;          // 0: Y, 1: Z
+    double c, s;       // cos(theta/2), sin(theta/2)
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_adj_rot(cx<T> *__restrict__ psi, cx<T> *__restrict__ lam,
+                                                    const AdjRotArgs a, double *__restrict__ part) {
+    double acc[1] = {0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    const T c = (T)a.c, s = (T)a.s;
+    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < a.n_pairs; p += stride) {
+        const uint64_t i0 = insert_zero(p, a.q), i1 = i0 | (1ULL << a.q);
+        const cx<T> x0 = psi[i0], x1 = psi[i1], l0 = lam[i0], l1 = lam[i1];
+        const double x0r = x0.re, x0i = x0.im, x1r = x1.re, x1i = x1.im;
+        const double l0r = l0.re, l0i = l0.im, l1r = l1.re, l1i = l1.im;
+        if (a.axis == 0) {
+            // <l|Y x> = conj(l0)(-i x1) + conj(l1)(i x0); Im of it
+            // conj(l0)(-i x1): Im = -(l0r*x1r + l0i*x1i)
+            // conj(l1)( i x0): Im =  (l1r*x0r + l1i*x0i)
+            acc[0] += (l1r * x0r + l1i * x0i) - (l0r * x1r + l0i * x1i);
+            // RY(theta)^dag = [[c, s], [-s, c]]
+            psi[i0] = {c * x0.re + s * x1.re, c * x0.im + s * x1.im};
+            psi[i1] = {c * x1.re - s * x0.re, c * x1.im - s * x0.im};
+            lam[i0] = {c * l0.re + s * l1.re, c * l0.im + s * l1.im};
+            lam[i1] = {c * l1.re - s * l0.re, c * l1.im - s * l0.im};
+        } else {
+            // <l|Z x> = conj(l0) x0 - conj(l1) x1; Im(conj(l) x) = lr*xi - li*xr
+            acc[0] += (l0r * x0i - l0i * x0r) - (l1r * x1i - l1i * x1r);
+            // RZ(theta)^dag = diag(e^{+i theta/2}, e^{-i theta/2})
+            const cx<T> ep = {c, s}, en = {c, -s};
+            psi[i0] = cmul(x0, ep);
+            psi[i1] = cmul(x1, en);
+            lam[i0] = cmul(l0, ep);
+            lam[i1] = cmul(l1, en);
+        }
+    }
+    block_sum<1>(acc, part + blockIdx.x);
+}
+
+// CNOT (self-inverse) on two vectors: swap target 0/1 where the control is 1.
+struct AdjCnotArgs {
+    uint64_t n_quads;
+    int ctrl, targ;
+    int lo, hi;        // min/max of (ctrl, targ)
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cnot2(cx<T> *__restrict__ x, cx<T> *__restrict__ y, const AdjCnotArgs a) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < a.n_quads; p += stride) {
+        const uint64_t i = insert_zero(insert_zero(p, a.lo), a.hi) | (1ULL << a.ctrl);
+        const uint64_t j = i | (1ULL << a.targ);
+        const cx<T> u = x[i];
+        x[i] = x[j];
+        x[j] = u;
+        if (y) {
+            const cx<T> v = y[i];
+            y[i] = y[j];
+            y[j] = v;
+        }
+    }
+}
+
+// lambda = H psi with H = sum_i Z_i Z_{i+1 mod n} (diagonal).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_apply_zz_ring(const cx<T> *__restrict__ psi, cx<T> *__restrict__ lam,
+                                                          uint64_t n_amp, int n) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    const uint64_t mask = (n >= 64) ? ~0ULL : ((1ULL << n) - 1);
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n_amp; i += stride) {
+        const uint64_t rot = ((i >> 1) | (i << (n - 1))) & mask;
+        const T zz = (T)(n - 2 * __popcll((i ^ rot) & mask));
+        lam[i] = {zz * psi[i].re, zz * psi[i].im};
+    }
+}
+
+}  // namespace sqk

@@ -232,27 +232,17 @@ extern "C" int sq_apply_blocks(sq_engine *e, const sq_gate *gates, const uint32_
         int rc = e->lower(gates[i], low[i]);
         if (rc) { e->err += " (gate " + std::to_string(i) + ")"; return rc; }
     }
+    if (e->out_of_core())   // same map as the gates in order
+        return e->stream_apply(std::vector<sq_gate>(gates, gates + total), low);
     return e->dtype == SQ_C128 ? run_blocks<double>(e, gates, low, block_sizes, n_blocks)
                                : run_blocks<float>(e, gates, low, block_sizes, n_blocks);
 }
 
 void sq_engine::free_extras() {
-    if (d_blockprog) hipFree(d_blockprog);
-    if (d_fused) hipFree(d_fused);
-    if (d_aux) hipFree(d_aux);
-    if (d_ckpt) hipFree(d_ckpt);
-    d_blockprog = d_fused = d_aux = d_ckpt = nullptr;
+    for (void **p : {&d_blockprog, &d_fused, &d_aux, &d_ckpt, &d_gradpart}) {
+        if (*p) hipFree(*p);
+        *p = nullptr;
+    }
+    blockprog_cap = fused_cap = aux_cap = ckpt_cap = gradpart_cap = 0;
 }
 
-extern "C" int sq_apply_fused(sq_engine *e, const sq_fused *, uint64_t) {
-    return e ? e->fail(SQ_ERR_UNSUPPORTED, "fused blocks not built yet") : SQ_ERR_ARG;
-}
-extern "C" int sq_compose_block(sq_engine *e, const sq_gate *, uint64_t, int32_t, const int32_t *, double *) {
-    return e ? e->fail(SQ_ERR_UNSUPPORTED, "compose not built yet") : SQ_ERR_ARG;
-}
-extern "C" int sq_hea_energy(sq_engine *e, int32_t, const double *, double *) {
-    return e ? e->fail(SQ_ERR_UNSUPPORTED, "adjoint not built yet") : SQ_ERR_ARG;
-}
-extern "C" int sq_adjoint_grad_hea(sq_engine *e, int32_t, const double *, int32_t, double *, double *) {
-    return e ? e->fail(SQ_ERR_UNSUPPORTED, "adjoint not built yet") : SQ_ERR_ARG;
-}

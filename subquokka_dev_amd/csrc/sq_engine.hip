@@ -276,7 +276,6 @@ int sq_create(const sq_config *cfg, sq_engine **out) {
     // a resident chunk at least as large as the state means the state fits in HBM
     const int resident = cfg->resident_qbit >= cfg->total_qbit ? 0 : cfg->resident_qbit;
     if (resident < 0) return SQ_ERR_ARG;
-    if (resident != 0) return SQ_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) return SQ_ERR_HIP;
     auto *e = new sq_engine();
@@ -286,8 +285,13 @@ int sq_create(const sq_config *cfg, sq_engine **out) {
     e->device = cfg->device;
     if (hipSetDevice(e->device) != hipSuccess) { delete e; return SQ_ERR_HIP; }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { delete e; return SQ_ERR_HIP; }
-    const size_t bytes = e->state_bytes();
-    if (hipMalloc(&e->d_state, bytes) != hipSuccess) {
+    if (resident != 0) {
+        // out-of-core runner: resident chunk of 2^resident amplitudes
+        if (resident < 8 || resident > 34) { hipStreamDestroy(e->stream); delete e; return SQ_ERR_ARG; }
+        e->res_qbit = resident;
+        int rc = e->stream_create();
+        if (rc) { e->stream_destroy(); hipStreamDestroy(e->stream); delete e; return rc; }
+    } else if (hipMalloc(&e->d_state, e->state_bytes()) != hipSuccess) {
         hipGetLastError();
         hipStreamDestroy(e->stream);
         delete e;
@@ -308,6 +312,7 @@ int sq_destroy(sq_engine *e) {
     hipSetDevice(e->device);
     if (e->stream) hipStreamSynchronize(e->stream);
     sq_stats_reset(e);
+    if (e->out_of_core()) e->stream_destroy();
     if (e->d_state) hipFree(e->d_state);
     if (e->d_part) hipFree(e->d_part);
     if (e->d_stage) hipFree(e->d_stage);
@@ -322,6 +327,7 @@ const char *sq_last_error(const sq_engine *e) { return e ? e->err.c_str() : "nul
 int sq_init_zero(sq_engine *e) {
     if (!e) return SQ_ERR_ARG;
     SQ_HIP(hipSetDevice(e->device));
+    if (e->out_of_core()) return e->stream_init_zero();
     const uint64_t n = e->n_amps();
     if (e->dtype == SQ_C128)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fill_zero_one<double>), e->grid_for(n, kBlock), kBlock, 0, e->stream, reinterpret_cast<cx<double> *>(e->d_state), n);
@@ -338,6 +344,7 @@ int sq_synchronize(sq_engine *e) {
 
 int sq_device_state(sq_engine *e, void **dptr, void **stream) {
     if (!e) return SQ_ERR_ARG;
+    if (e->out_of_core()) return e->fail(SQ_ERR_ARG, "the out-of-core state is in host memory");
     if (dptr) *dptr = e->d_state;
     if (stream) *stream = (void *)e->stream;
     return SQ_OK;
@@ -350,6 +357,7 @@ int sq_set_state(sq_engine *e, const void *host, uint64_t n_amps, int32_t dtype)
     if (n_amps != e->n_amps()) return e->fail(SQ_ERR_ARG, "sq_set_state: n_amps must be 2^total_qbit");
     if (dtype != SQ_C128 && dtype != SQ_C64) return e->fail(SQ_ERR_ARG, "bad dtype");
     SQ_HIP(hipSetDevice(e->device));
+    if (e->out_of_core()) return e->stream_set_state(host, dtype);
     if (dtype == e->dtype) {
         SQ_HIP(hipMemcpyAsync(e->d_state, host, e->state_bytes(), hipMemcpyHostToDevice, e->stream));
         SQ_HIP(hipStreamSynchronize(e->stream));
@@ -377,6 +385,7 @@ int sq_get_state(sq_engine *e, void *host, uint64_t n_amps, int32_t dtype) {
     if (n_amps != e->n_amps()) return e->fail(SQ_ERR_ARG, "sq_get_state: n_amps must be 2^total_qbit");
     if (dtype != SQ_C128 && dtype != SQ_C64) return e->fail(SQ_ERR_ARG, "bad dtype");
     SQ_HIP(hipSetDevice(e->device));
+    if (e->out_of_core()) return e->stream_get_state(host, dtype);
     if (dtype == e->dtype) {
         SQ_HIP(hipMemcpyAsync(host, e->d_state, e->state_bytes(), hipMemcpyDeviceToHost, e->stream));
         SQ_HIP(hipStreamSynchronize(e->stream));
@@ -408,6 +417,7 @@ int sq_apply_gates(sq_engine *e, const sq_gate *gates, uint64_t n) {
         int rc = e->lower(gates[i], low[i]);
         if (rc) { e->err += " (gate " + std::to_string(i) + ")"; return rc; }
     }
+    if (e->out_of_core()) return e->stream_apply(std::vector<sq_gate>(gates, gates + n), low);
     for (uint64_t i = 0; i < n; i++) {
         int rc = e->apply_lowered(low[i]);
         if (rc) return rc;
@@ -494,6 +504,7 @@ int sq_stats_get(sq_engine *e, sq_kstat *out, int32_t cap, int32_t *n_out) {
 // ---------------------------------------------------------------------------
 
 int sq_engine::norm_zz(double *v) {
+    if (out_of_core()) return stream_norm_zz(v);
     int rc = ensure_partials();
     if (rc) return rc;
     hipSetDevice(device);
@@ -516,6 +527,7 @@ int sq_engine::norm_zz(double *v) {
 
 int sq_engine::compare(sq_engine *o, double *v) {
     if (o->n != n || o->device != device) return fail(SQ_ERR_ARG, "engines differ in size or device");
+    if (out_of_core() || o->out_of_core()) return fail(SQ_ERR_UNSUPPORTED, "compare needs HBM-resident states");
     int rc = ensure_partials();
     if (rc) return rc;
     hipSetDevice(device);

@@ -70,6 +70,7 @@ _SIGS = {
     "sq_apply_gates": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.c_uint64]),
     "sq_apply_blocks": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64]),
     "sq_apply_fused": (ctypes.c_int, [_P, ctypes.POINTER(SqFused), ctypes.c_uint64]),
+    "sq_run_fused": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64]),
     "sq_compose_block": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.c_uint64, ctypes.c_int32,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
     "sq_norm2": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
@@ -254,6 +255,10 @@ class Engine:
                 arr[i].q[j] = q
             arr[i].u = u.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         self._c(lib().sq_apply_fused(self.h, arr, len(blocks)))
+
+    def run_fused(self, gates, blocks, n_blocks):
+        """Planner blocks (<= 5 qubits each): FP64 composition + MFMA application."""
+        self._c(lib().sq_run_fused(self.h, gates, blocks, n_blocks))
 
     def compose(self, gates, n, qubits):
         k = len(qubits)

@@ -1,9 +1,11 @@
 // Driver of the CPU emulation build: runs one circuit through the C ABI with
 // the engine compiled for the host (see hip/hip_runtime.h in this folder).
-//   emul <plain|sub> <n> <chunk> <c128|c64> <circuit file>
+//   emul <plain|sub|fused> <n> <chunk> <c128|c64> <circuit file>
+// (fused: subcircuit-format file whose blocks act on at most five qubits)
 // Exit code 0 means every launch stayed in bounds (AddressSanitizer aborts
 // the process on the first bad access).
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <sstream>
 #include <string>
@@ -14,8 +16,10 @@
 namespace sqk { alignas(16) unsigned char lds_raw[1 << 16]; }
 
 int main(int argc, char **argv) {
-    if (argc != 6) { std::fprintf(stderr, "usage: emul plain|sub n chunk c128|c64 file\n"); return 2; }
-    const bool sub = std::string(argv[1]) == "sub";
+    if (argc != 6 && argc != 7) { std::fprintf(stderr, "usage: emul plain|sub|fused n chunk c128|c64 file [out]\n"); return 2; }
+    const std::string mode = argv[1];
+    const bool sub = mode == "sub" || mode == "fused";
+    const bool ooc = mode == "ooc";   // out-of-core: chunk argument = resident_qbit
     const int n = std::atoi(argv[2]), chunk = std::atoi(argv[3]);
     const int dtype = std::string(argv[4]) == "c64" ? SQ_C64 : SQ_C128;
     std::ifstream f(argv[5]);
@@ -31,13 +35,39 @@ int main(int argc, char **argv) {
                               sub ? nb : 0, &nb, &line);
     if (rc) { std::fprintf(stderr, "parse rc=%d line %d\n", rc, line); return 3; }
     sq_config cfg{};
-    cfg.total_qbit = n; cfg.chunk_qbit = chunk; cfg.dtype = dtype;
+    cfg.total_qbit = n; cfg.chunk_qbit = ooc ? 0 : chunk; cfg.dtype = dtype;
+    if (ooc) cfg.resident_qbit = chunk;
     sq_engine *e = nullptr;
+    const char *seed_env = std::getenv("EMUL_INIT_SEED");
     if ((rc = sq_create(&cfg, &e))) { std::fprintf(stderr, "create rc=%d\n", rc); return 4; }
-    rc = sub ? sq_apply_blocks(e, gates.data(), sizes.data(), nb) : sq_apply_gates(e, gates.data(), ng);
+    if (mode == "adjoint") {   // chunk argument = layers; theta from a fixed formula
+        const int layers = chunk;
+        std::vector<double> theta((size_t)2 * n * layers), grad(theta.size());
+        for (size_t i = 0; i < theta.size(); i++) theta[i] = 0.1 + 0.37 * (double)i;
+        double energy = 0;
+        rc = sq_adjoint_grad_hea(e, layers, theta.data(), 1, &energy, grad.data());
+        if (rc) { std::fprintf(stderr, "adjoint rc=%d %s\n", rc, sq_last_error(e)); return 10; }
+        rc = sq_hea_energy(e, layers, theta.data(), &energy);
+        if (rc) return 11;
+        sq_destroy(e);
+        std::printf("ok adjoint params=%zu\n", theta.size());
+        return 0;
+    }
+    if (seed_env) {   // seeded Gaussian initial state instead of |0...0>
+        std::vector<double> st(2ULL << n);
+        sq_gen_state(n, std::strtoull(seed_env, nullptr, 10), st.data());
+        if ((rc = sq_set_state(e, st.data(), 1ULL << n, SQ_C128))) return 9;
+    }
+    if (mode == "fused") rc = sq_run_fused(e, gates.data(), sizes.data(), nb);
+    else rc = sub ? sq_apply_blocks(e, gates.data(), sizes.data(), nb) : sq_apply_gates(e, gates.data(), ng);
     if (rc) { std::fprintf(stderr, "apply rc=%d %s\n", rc, sq_last_error(e)); return 5; }
     std::vector<double> out(2ULL << n);
     if ((rc = sq_get_state(e, out.data(), 1ULL << n, SQ_C128))) { std::fprintf(stderr, "get rc=%d\n", rc); return 6; }
+    if (argc == 7) {   // optional state dump (interleaved doubles)
+        FILE *fo = std::fopen(argv[6], "wb");
+        if (!fo || std::fwrite(out.data(), sizeof(double), out.size(), fo) != out.size()) return 8;
+        std::fclose(fo);
+    }
     double nrm = 0;
     if ((rc = sq_norm2(e, &nrm))) return 7;
     sq_destroy(e);

@@ -58,6 +58,9 @@ inline hipError_t hipMalloc(void **p, size_t n) {
 }
 template <typename T> inline hipError_t hipMalloc(T **p, size_t n) { return hipMalloc(reinterpret_cast<void **>(p), n); }
 inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
+enum { hipHostMallocDefault = 0 };
+inline hipError_t hipHostMalloc(void **p, size_t n, unsigned) { return hipMalloc(p, n); }
+inline hipError_t hipHostFree(void *p) { std::free(p); return hipSuccess; }
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memcpy(d, s, n); return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t) {
     return hipMemcpy(d, s, n, k);
@@ -70,6 +73,10 @@ inline hipError_t hipEventCreate(hipEvent_t *e) { *e = new sq_emul_event; return
 inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0; return hipSuccess; }
+
+// Matrix-core instructions are wave-wide; here they leave the accumulator as is.
+#define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) ((void)(a), (void)(b), (c))
+#define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) ((void)(a), (void)(b), (c))
 
 template <typename F, typename... A>
 void sq_emul_launch(F kernel, dim3 grid, dim3 block, size_t, hipStream_t, A... args) {

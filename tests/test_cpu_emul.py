@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 import subquokka_dev_amd as sq
@@ -28,14 +29,19 @@ def emul():
     return EMUL
 
 
-def run_case(emul, tmp_path, mode, n, chunk, dtype, text):
+def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None):
     f = tmp_path / "c.txt"
     f.write_text(text)
+    out = tmp_path / "state.bin"
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
-    r = subprocess.run([emul, mode, str(n), str(chunk), dtype, str(f)], capture_output=True, text=True,
+    if seed is not None:
+        env["EMUL_INIT_SEED"] = str(seed)
+    r = subprocess.run([emul, mode, str(n), str(chunk), dtype, str(f), str(out)], capture_output=True, text=True,
                        timeout=900, env=env)
     assert r.returncode == 0, (r.stdout + r.stderr)[-3000:]
     assert r.stdout.startswith("ok")
+    raw = np.fromfile(out, dtype=np.float64)
+    return raw[0::2] + 1j * raw[1::2]
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
@@ -70,3 +76,39 @@ def test_finder_plan_with_vswaps(emul, tmp_path, dtype):
 def test_plain_all_gate_kinds(emul, tmp_path, dtype, n):
     text = random_circuit(n, 120, seed=n) + "VSWAP_1_1 0 %d\nVSWAP_2_2 0 1 %d %d\n" % (n - 1, n - 2, n - 1)
     run_case(emul, tmp_path, "plain", n, n, dtype, text)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [6, 9, 12])
+def test_fused_blocks(emul, tmp_path, dtype, n):
+    text = sq.plan(random_circuit(n, 150, seed=n + 1), n, 1, 5)
+    run_case(emul, tmp_path, "fused", n, n, dtype, text)
+
+
+def test_fused_small_state_numerics(emul, tmp_path, oracle):
+    # Below 9 qubits the fused path is scalar code end to end (FP64 composition,
+    # one thread per column), so the emulation computes it exactly.
+    n = 7
+    plain = random_circuit(n, 120, seed=17)
+    text = sq.plan(plain, n, 1, 5)
+    got = run_case(emul, tmp_path, "fused", n, n, "c128", text, seed=3)
+    want = oracle.run(sq.gen_state(n, 3), n, plain)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n,layers", [(2, 1), (6, 3), (12, 2)])
+def test_adjoint_launches(emul, tmp_path, dtype, n, layers):
+    f = tmp_path / "empty.txt"
+    f.write_text("")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    r = subprocess.run([emul, "adjoint", str(n), str(layers), dtype, str(f)], capture_output=True, text=True,
+                       timeout=900, env=env)
+    assert r.returncode == 0 and r.stdout.startswith("ok"), (r.stdout + r.stderr)[-3000:]
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n,w", [(12, 9), (14, 10), (13, 12)])
+def test_out_of_core_launches(emul, tmp_path, dtype, n, w):
+    text = sq.gen_circuit("random", n, 3, seed=n) + random_circuit(n, 60, seed=w)
+    run_case(emul, tmp_path, "ooc", n, w, dtype, text)
