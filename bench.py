@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--cpu-sample-gates", type=int, default=45)
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--fused-k", type=int, default=5)
+    ap.add_argument("--fused-dtypes", default="c64,c128")
+    ap.add_argument("--no-adjoint", action="store_true")
     return ap.parse_args()
 
 
@@ -164,7 +166,7 @@ def main():
         fg, fb, fng, fnb = sq.parse_circuit(plan, n, is_subcircuit=True)
         print(f"[bench] fused plan: {fnb} blocks of <= {args.fused_k} qubits ({plan_s:.2f} s)", file=sys.stderr, flush=True)
         fused = {}
-        for dtype in ("c64", "c128"):
+        for dtype in args.fused_dtypes.split(","):
             eng = sq.Engine(n, dtype=dtype)
             for _ in range(args.warmup):
                 eng.init_zero()
@@ -214,6 +216,31 @@ def main():
                             "infidelity_vs_c128": infid, "max_abs_err_vs_c128": eng128.max_abs_diff(eng64)}
         eng64.close()
     eng128.close()
+
+    if not args.no_adjoint:
+        # Adjoint gradient of the 24-qubit hardware-efficient ansatz (8 layers,
+        # 384 parameters), complex64 with FP64 checkpoints every 3 layers:
+        # cost of energy + full gradient against one forward pass.
+        an, al = 24, 8
+        theta = np.random.default_rng(4).uniform(-np.pi, np.pi, 2 * an * al)
+        ea = sq.Engine(an, dtype="c64")
+        ea.hea_energy(al, theta)
+        ea.adjoint_grad_hea(al, theta, 3)
+        ea.sync()
+        fwd, grd = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ea.hea_energy(al, theta)
+            fwd.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            _, g = ea.adjoint_grad_hea(al, theta, 3)
+            grd.append(time.perf_counter() - t0)
+        ea.close()
+        f, gtime = min(fwd), min(grd)
+        out["adjoint_24q_384p_c64"] = {"forward_ms": round(f * 1e3, 3), "energy_and_gradient_ms": round(gtime * 1e3, 3),
+                                       "gradient_over_forward": round(gtime / f, 3), "checkpoint_every_layers": 3}
+        print(f"[bench] adjoint 24q: forward {f*1e3:.2f} ms, energy+gradient {gtime*1e3:.2f} ms",
+              file=sys.stderr, flush=True)
 
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

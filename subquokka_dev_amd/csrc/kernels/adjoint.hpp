@@ -87,7 +87,8 @@ __global__ void __launch_bounds__(kBlock) k_apply_zz_ring(const cx<T> *__restric
     const uint64_t mask = (n >= 64) ? ~0ULL : ((1ULL << n) - 1);
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n_amp; i += stride) {
         const uint64_t rot = ((i >> 1) | (i << (n - 1))) & mask;
-        const T zz = (T)(n - 2 * __popcll((i ^ rot) & mask));
+        const int differ = (int)__popcll((i ^ rot) & mask);   // __popcll is unsigned: keep the sum signed
+        const T zz = (T)(n - 2 * differ);
         lam[i] = {zz * psi[i].re, zz * psi[i].im};
     }
 }

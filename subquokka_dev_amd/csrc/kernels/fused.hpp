@@ -10,14 +10,15 @@
 // real form [Yr; Yi] = [[Ur, -Ui], [Ui, Ur]] [Xr; Xi]: a 64x64 real matrix M
 // times 64 x 16 column batches, done with v_mfma_{f32,f64}_16x16x4 (K = 64 in
 // 16 steps, four 16-row output tiles).  Operand layout of the 16x16x4 MFMA
-// (lane l, g = l/16, c = l%16):  A[c][g], B[g][c], D[4g + r][c] for r = 0..3.
+// (lane l, g = l/16, c = l%16):  A[c][g], B[g][c]; D[4g + r][c] (f32) or
+// D[g + 4r][c] (f64) for r = 0..3 (both measured on gfx950).
 //   * A: lane l keeps M[16 rt + c][4 st + g] for rt < 4, st < 16 (64 values),
 //     loaded once per wave and reused for every batch.
 //   * B at step st is row 4 st + g of X: Re x_j (st < 8) or Im x_j (st >= 8)
 //     with j = 4 (st mod 8) + g, so lane l loads amplitudes j = g + 4t, t < 8,
 //     of column c.
-//   * D tile rt row 4g + r is real row 16 rt + 4g + r: lane l stores
-//     y_{4g+r} = (D0, D2) and y_{16+4g+r} = (D1, D3).
+//   * D tile rt holds row d = row(g, r) of its 16: real row 16 rt + d, so lane l
+//     stores y_d = (tile 0, tile 2) and y_{16+d} = (tile 1, tile 3).
 // Every column is read and written by exactly one wave, so the update is in place.
 #pragma once
 #include "common.hpp"
@@ -30,10 +31,12 @@ template <typename T> struct MfmaTypes;
 template <> struct MfmaTypes<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
     static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __host__ __device__ constexpr int row(int g, int r) { return 4 * g + r; }   // D row in register r
 };
 template <> struct MfmaTypes<double> {
     typedef double v4 __attribute__((ext_vector_type(4)));
     static __device__ inline v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __host__ __device__ constexpr int row(int g, int r) { return g + 4 * r; }
 };
 
 struct FusedArgs {
@@ -73,8 +76,8 @@ __global__ void __launch_bounds__(256) k_fused5(cx<T> *__restrict__ s, const Fus
     for (int t = 0; t < 8; t++) in_off[t] = pdep5(g + 4 * t, a.q);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        out_off[r] = pdep5(4 * g + r, a.q);
-        out_off[4 + r] = pdep5(16 + 4 * g + r, a.q);
+        out_off[r] = pdep5(MfmaTypes<T>::row(g, r), a.q);
+        out_off[4 + r] = pdep5(16 + MfmaTypes<T>::row(g, r), a.q);
     }
 
     for (uint64_t b0 = wave * U; b0 < a.n_batches; b0 += n_waves * U) {

@@ -25,7 +25,8 @@ def test_out_of_core_matches_oracle(oracle, dtype, n, w):
     want = oracle.run(s0, n, text)
     tol = 1e-12 if dtype == "c128" else 2e-6
     assert np.max(np.abs(got - want)) <= tol
-    assert abs(norm - 1) < (1e-12 if dtype == "c128" else 1e-5)
+    # H uses the reference's literal 0.70710678118, so the norm drifts from 1
+    assert abs(norm - np.vdot(want, want).real) < (1e-12 if dtype == "c128" else 1e-5)
 
 
 def test_out_of_core_24q_matches_hbm_engine():
