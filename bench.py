@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -42,6 +44,15 @@ def parse():
     ap.add_argument("--fused-k", type=int, default=5)
     ap.add_argument("--fused-dtypes", default="c64,c128")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--workload", default="random30",
+                    choices=["random30", "fused32", "stream", "qft20", "traffic"],
+                    help="random30: the headline line (default).  fused32: 32 qubits, 24 layers, "
+                         "fused complex64 vs plain complex128.  stream: out-of-core runner "
+                         "(state in pinned host memory) vs the same circuit in HBM.  qft20: "
+                         "20-qubit QFT on a seeded state.  traffic: a short fixed kernel "
+                         "sequence for rocprofv3 --pmc HBM-byte counts")
+    ap.add_argument("--stream-dtype", default="c128")
+    ap.add_argument("--resident-qbit", type=int, default=30)
     return ap.parse_args()
 
 
@@ -94,6 +105,207 @@ def cpu_baseline(text, n, sample_gates):
             "seconds": secs.value}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def line(metric, value, unit, steps, warmup, ms, dtype, config, **extra):
+    out = {"metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": round(ms, 2), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded generator)", "config": config}
+    out.update(extra)
+    return out
+
+
+def fused_kernel_figures(st, dtype, n):
+    k = st.get(f"fused5_{dtype}", {"launches": 1, "total_ms": float("nan"), "alg_bytes": 0})
+    avg_ms = k["total_ms"] / k["launches"]
+    bpl = k["alg_bytes"] / k["launches"]
+    flops = 8.0 * 32 * 2 ** n     # complex 32x32 per 32-amplitude column
+    peak_tf = 157.3 if dtype == "c64" else 78.6
+    return {"avg_launch_ms": round(avg_ms, 4), "hbm_gbs": round(bpl / (avg_ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(bpl / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "mfma_tflops": round(flops / (avg_ms * 1e-3) / 1e12, 1),
+            "mfma_frac": round(flops / (avg_ms * 1e-3) / 1e12 / peak_tf, 4), "mfma_peak_tflops": peak_tf}
+
+
+def workload_fused32(args):
+    """32 qubits, 24 layers of the random-layer generator (1152 gates, seed 3):
+    planner -> 5-qubit blocks composed in FP64 and applied on MFMA to a
+    complex64 state (32 GiB), against the plain complex128 sweep (64 GiB)."""
+    import subquokka_dev_amd as sq
+    n, layers, seed = 32, 24, 3
+    text = sq.gen_circuit("random", n, layers, seed=seed)
+    gates, _, ng, _ = sq.parse_circuit(text, n)
+    log(f"fused32: {ng} gates")
+    ref = sq.Engine(n, dtype="c128")
+    ref.init_zero()
+    ref.sync()
+    t0 = time.perf_counter()
+    ref.apply_gates(gates, ng)
+    ref.sync()
+    t_plain = time.perf_counter() - t0
+    log(f"plain c128: {t_plain:.2f} s")
+    t0 = time.perf_counter()
+    plan = sq.plan(text, n, 1, args.fused_k)
+    plan_s = time.perf_counter() - t0
+    fg, fb, _, fnb = sq.parse_circuit(plan, n, is_subcircuit=True)
+    eng = sq.Engine(n, dtype="c64")
+    for _ in range(args.warmup):
+        eng.init_zero()
+        eng.run_fused(fg, fb, fnb)
+    eng.sync()
+    eng.stats_reset()
+    eng.stats_enable(True)
+    times = []
+    for _ in range(args.steps):
+        eng.init_zero()
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.run_fused(fg, fb, fnb)
+        eng.sync()
+        times.append(time.perf_counter() - t0)
+        log(f"fused c64 step {len(times)}: {times[-1]*1e3:.1f} ms")
+    st = eng.stats()
+    eng.stats_enable(False)
+    ov = ref.inner(eng)
+    infid = 1.0 - abs(ov) ** 2 / (ref.norm2() * eng.norm2())
+    tt = sum(times) / len(times)
+    out = line("32q random circuit: fused c64 gates/s (planner blocks, MFMA) vs plain c128",
+               ng / tt, "gates/s", args.steps, args.warmup, tt * 1e3, "f32",
+               {"workload": f"random_layers_{n}q_{layers}l_seed{seed}_fused{args.fused_k}_c64", "n_qubits": n,
+                "layers": layers, "gates": ng, "blocks": fnb, "state": "complex64 in HBM (32 GiB)"},
+               plain_c128={"gates_s": round(ng / t_plain, 2), "seconds": round(t_plain, 3)},
+               speedup_vs_plain_c128=round(t_plain / tt, 2), infidelity_vs_plain_c128=infid,
+               max_abs_err_vs_plain_c128=ref.max_abs_diff(eng), plan_s=round(plan_s, 3),
+               kernel=fused_kernel_figures(st, "c64", n),
+               compose_ms_per_step=round(st.get("compose_f64", {"total_ms": 0})["total_ms"] / args.steps, 3))
+    eng.close()
+    ref.close()
+    return out
+
+
+def workload_stream(args):
+    """Out-of-core runner (DIO_Runner role, host RAM tier): the state lives in
+    pinned host memory and is streamed through two HBM buffers of
+    2^resident_qbit amplitudes, pass by pass; the same circuit then runs with
+    the state resident in HBM.  12 layers of the random-layer generator, seed 5."""
+    import subquokka_dev_amd as sq
+    n, layers, seed, dtype = args.qubits, 12, 5, args.stream_dtype
+    text = sq.gen_circuit("random", n, layers, seed=seed)
+    per_layer = [l for l in text.splitlines() if l.strip()]
+    ng = len(per_layer)
+    chunks = 4
+    step = (ng + chunks - 1) // chunks
+    parts = ["\n".join(per_layer[i:i + step]) + "\n" for i in range(0, ng, step)]
+    B = 16 if dtype == "c128" else 8
+    log(f"stream: {n} qubits {dtype}, {ng} gates, host state {B * 2 ** n / 2 ** 30:.0f} GiB, "
+        f"HBM groups of 2^{args.resident_qbit}")
+    eng = sq.Engine(n, dtype=dtype, resident_qbit=args.resident_qbit)
+    t0 = time.perf_counter()
+    eng.init_zero()
+    eng.sync()
+    log(f"host init {time.perf_counter() - t0:.1f} s")
+    eng.stats_reset()
+    t0 = time.perf_counter()
+    for i, p in enumerate(parts):
+        eng.apply_text(p)
+        eng.sync()
+        log(f"stream part {i + 1}/{len(parts)}: {time.perf_counter() - t0:.1f} s")
+    t_stream = time.perf_counter() - t0
+    passes, link_bytes = eng.stream_info()
+    zz_s, nrm_s = eng.expect_zz_ring(), eng.norm2()
+    eng.close()
+    hbm = sq.Engine(n, dtype=dtype)
+    hbm.init_zero()
+    hbm.sync()
+    t0 = time.perf_counter()
+    hbm.apply_text(text)
+    hbm.sync()
+    t_hbm = time.perf_counter() - t0
+    zz_h, nrm_h = hbm.expect_zz_ring(), hbm.norm2()
+    hbm.close()
+    log(f"in HBM: {t_hbm:.2f} s; <ZZ> {zz_s:.12f} vs {zz_h:.12f}")
+    return line(f"{n}q random circuit, state in pinned host memory streamed through HBM: gates/s",
+                ng / t_stream, "gates/s", 1, 0, t_stream * 1e3, "f64" if dtype == "c128" else "f32",
+                {"workload": f"random_layers_{n}q_{layers}l_seed{seed}_stream_{dtype}_w{args.resident_qbit}",
+                 "n_qubits": n, "layers": layers, "gates": ng, "state": f"{dtype} in pinned host memory"},
+                passes=passes, host_link_gbs=round(link_bytes / t_stream / 1e9, 1),
+                host_link_bytes=link_bytes, in_hbm={"seconds": round(t_hbm, 3), "gates_s": round(ng / t_hbm, 2)},
+                stream_over_hbm_time=round(t_stream / t_hbm, 2),
+                zz_ring={"stream": zz_s, "hbm": zz_h, "abs_diff": abs(zz_s - zz_h)},
+                norm2={"stream": nrm_s, "hbm": nrm_h})
+
+
+def workload_qft20(args):
+    """20-qubit QFT (cir_qft.py order, 220 gates) on a seeded Gaussian state,
+    complex128, whole state in HBM (16 MiB: cache-resident, launch-bound)."""
+    import subquokka_dev_amd as sq
+    n = 20
+    text = sq.gen_circuit("qft", n)
+    gates, _, ng, _ = sq.parse_circuit(text, n)
+    s0 = sq.gen_state(n, 1)
+    eng = sq.Engine(n, dtype="c128")
+    for _ in range(args.warmup):
+        eng.set_state(s0)
+        eng.apply_gates(gates, ng)
+    times = []
+    for _ in range(args.steps):
+        eng.set_state(s0)
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.apply_gates(gates, ng)
+        eng.sync()
+        times.append(time.perf_counter() - t0)
+    eng.close()
+    tt = sum(times) / len(times)
+    return line("20q QFT on a seeded state: gates/s", ng / tt, "gates/s", args.steps, args.warmup, tt * 1e3,
+                "f64", {"workload": "qft_20q_seed1_plain_c128", "n_qubits": n, "gates": ng,
+                        "state": "complex128 in HBM (16 MiB)"})
+
+
+def workload_traffic(args):
+    """A short fixed kernel sequence whose HBM bytes per dispatch rocprofv3
+    --pmc reads (FETCH_SIZE, WRITE_SIZE): a 4 GiB device copy of known size
+    (the scale factor), then on a 30-qubit complex128 state 3 dense one-qubit
+    gates with the target at or above bit 6, 3 with the target in bits 0-5,
+    3 CP gates, and one fused 5-qubit block in complex128 and in complex64."""
+    import torch
+    import subquokka_dev_amd as sq
+    n = 30
+    a = torch.ones(2 ** 30, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(5):
+        b.copy_(a)
+    ev1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 5 * 2 * 4 * 2 ** 30 / (ev0.elapsed_time(ev1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    u = "0.6 0.0 0.0 0.8 0.8 0.0 0.0 -0.6"
+    text = "".join(f"U1 {q} {u}\n" for q in (20, 25, 29)) + "".join(f"U1 {q} {u}\n" for q in (0, 2, 5)) + \
+        "".join(f"CP {a} {b} 3.141592653589793\n" for a, b in ((3, 17), (8, 22), (11, 28)))
+    eng = sq.Engine(n, dtype="c128")
+    eng.init_zero()
+    eng.apply_text(text)
+    eng.sync()
+    eng.close()
+    blk = "".join(f"U1 {q} {u}\n" for q in (6, 12, 18, 24, 29)) + "CP 6 29 0.5\n"
+    import ctypes
+    g, _, ng_, _ = sq.parse_circuit(blk, n)
+    for dtype in ("c128", "c64"):
+        eng = sq.Engine(n, dtype=dtype)
+        eng.init_zero()
+        eng.run_fused(g, (ctypes.c_uint32 * 1)(ng_), 1)
+        eng.sync()
+        eng.close()
+    return {"workload": "traffic_probe", "copy_gbs_torch": round(copy_gbs, 1)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -101,6 +313,13 @@ def main():
                           "(multi-GPU sharding is a design note, not an implemented path)"}))
         return 2
     import numpy as np
+    if args.workload != "random30":
+        fn = {"fused32": workload_fused32, "stream": workload_stream, "qft20": workload_qft20,
+              "traffic": workload_traffic}[args.workload]
+        if args.workload == "stream" and args.qubits == N_QUBITS:
+            args.qubits = 33
+        print(json.dumps(fn(args)), flush=True)
+        return 0
     import subquokka_dev_amd as sq
 
     n, layers = args.qubits, args.layers

@@ -160,6 +160,10 @@ typedef struct sq_kstat {
 int sq_stats_enable(sq_engine *e, int32_t on);
 int sq_stats_get(sq_engine *e, sq_kstat *out, int32_t cap, int32_t *n_out);
 int sq_stats_reset(sq_engine *e);
+/* Out-of-core runner telemetry (no reference counterpart; DIO_Runner.cpp has
+ * only the "<t>s" line): streamed passes since the last sq_stats_reset and the
+ * bytes those passes moved over the host link (both directions). */
+int sq_stream_info(sq_engine *e, uint64_t *passes, double *host_link_bytes);
 
 /* ---- host-side formats (no device needed) ---- */
 /* Parses one gate line of the MEM grammar (setGate_MEM, simulator.cpp:542-597;

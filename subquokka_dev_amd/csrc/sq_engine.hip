@@ -471,6 +471,15 @@ int sq_stats_reset(sq_engine *e) {
     hipStreamSynchronize(e->stream);
     for (auto &p : e->pending) { hipEventDestroy(p.start); hipEventDestroy(p.stop); }
     e->pending.clear();
+    e->n_stream_passes = 0;
+    return SQ_OK;
+}
+
+int sq_stream_info(sq_engine *e, uint64_t *passes, double *host_link_bytes) {
+    if (!e) return SQ_ERR_ARG;
+    if (passes) *passes = e->n_stream_passes;
+    // every pass moves the whole state host -> HBM and back
+    if (host_link_bytes) *host_link_bytes = 2.0 * (double)e->n_stream_passes * (double)e->n_amps() * (double)e->amp_bytes();
     return SQ_OK;
 }
 

@@ -84,6 +84,7 @@ _SIGS = {
     "sq_stats_enable": (ctypes.c_int, [_P, ctypes.c_int32]),
     "sq_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(SqKstat), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "sq_stats_reset": (ctypes.c_int, [_P]),
+    "sq_stream_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]),
     "sq_parse_gate": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(SqGate)]),
     "sq_parse_circuit": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(SqGate),
                                         ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
@@ -300,6 +301,12 @@ class Engine:
 
     def stats_reset(self):
         self._c(lib().sq_stats_reset(self.h))
+
+    def stream_info(self):
+        """(passes, host-link bytes) of the out-of-core runner since stats_reset."""
+        p, b = ctypes.c_uint64(), ctypes.c_double()
+        self._c(lib().sq_stream_info(self.h, ctypes.byref(p), ctypes.byref(b)))
+        return p.value, b.value
 
     def stats(self):
         arr = (SqKstat * 64)()
