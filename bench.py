@@ -105,6 +105,28 @@ def cpu_baseline(text, n, sample_gates):
             "seconds": secs.value}
 
 
+def copy_ceiling(eng, n):
+    """Device-to-device copy bandwidth (HIP's own copy kernel) inside the
+    engine's state buffer: the first half copied onto the second half, 5 times.
+    Bytes = read + write.  Measured before the timed steps; the state is reset
+    afterwards by init_zero."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpyDtoD.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    hip.hipDeviceSynchronize.argtypes = []
+    ptr, _ = eng.device_state()
+    half = (16 << n) // 2
+    eng.sync()
+    if hip.hipMemcpyDtoD(ptr + half, ptr, half) != 0 or hip.hipDeviceSynchronize() != 0:
+        return None
+    t0 = time.perf_counter()
+    for _ in range(5):
+        hip.hipMemcpyDtoD(ptr + half, ptr, half)
+    if hip.hipDeviceSynchronize() != 0:
+        return None
+    return 5 * 2 * half / (time.perf_counter() - t0) / 1e9
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -329,6 +351,8 @@ def main():
 
     def run_steps(dtype, steps, warmup, stats):
         eng = sq.Engine(n, dtype=dtype)
+        if dtype == "c128":
+            out_extra["hbm_copy_gbs"] = copy_ceiling(eng, n)
         for _ in range(warmup):
             eng.init_zero()
             eng.apply_gates(gates, ng)
@@ -348,6 +372,7 @@ def main():
         eng.stats_enable(False)
         return eng, times, st
 
+    out_extra = {}
     eng128, t128, st128 = run_steps("c128", args.steps, args.warmup, True)
     total_t = sum(t128)
     gates_s = args.steps * ng / total_t
@@ -374,6 +399,10 @@ def main():
                       "layers": layers, "gates": ng, "state": "complex128 in HBM",
                       "mode": "plain gate-by-gate (MEM_Runner::run(vector<Gate*>&) form)"},
            "hbm_gbs": round(hbm_gbs, 1), "roofline": roofline, "kernels_c128": kernels}
+    if out_extra.get("hbm_copy_gbs"):
+        out["hbm_copy_ceiling"] = {"gbs": round(out_extra["hbm_copy_gbs"], 1),
+                                   "what": "hipMemcpyDtoD of 8 GiB inside the state buffer (read + write bytes)",
+                                   "dominant_kernel_over_copy": round(achieved / out_extra["hbm_copy_gbs"], 4)}
 
     if not args.no_fused:
         # Planner (host, once per circuit, outside the timed region -- the

@@ -1,5 +1,6 @@
 // simulator.cpp -- see simulator.hpp.
 #include "simulator.hpp"
+#include "planner.hpp"
 
 #include <algorithm>
 #include <cctype>
@@ -145,6 +146,8 @@ void Simulator::setup_ini(const std::string &ini) {
     env_.device = (int)reader.get_int(s, "device", 0);
     if (env_.precision != "c128" && env_.precision != "c64")
         throw SimError(SQ_ERR_ARG, "[Config File]: precision must be c128 or c64");
+    if (env_.fused_k < 0 || env_.fused_k > 5)
+        throw SimError(SQ_ERR_ARG, "[Config File]: fused_k must be 0 (off) or 1..5");
     if (seg_.mpi > 0)
         throw SimError(SQ_ERR_UNSUPPORTED, "[Config File]: mpi_qbit > 0 needs several GPUs; this engine runs on one");
 
@@ -266,8 +269,21 @@ void HBM_Runner::check(int rc, const char *what) {
     if (rc != SQ_OK) throw SimError(rc, std::string(what) + ": " + sq_last_error(e_));
 }
 
-// MEM_Runner::run(vector<Gate*>&) (MEM_Runner.cpp:101-134)
+// MEM_Runner::run(vector<Gate*>&) (MEM_Runner.cpp:101-134).  With fused_k > 0
+// (state in HBM) the planner groups the gates into blocks of at most fused_k
+// qubits, each composed in FP64 and applied in one MFMA pass; the planning is
+// inside the timed region, like every other host step of the run.
 void HBM_Runner::run(std::vector<sq_gate> &circuit) {
+    if (env_.fused_k > 0 && !streamed_) {
+        std::vector<sq_gate> order;
+        std::vector<uint32_t> sizes;
+        for (const auto &p : sq::plan_passes(circuit, n_, env_.fused_k, {})) {
+            for (int gi : p.gates) order.push_back(circuit[gi]);
+            sizes.push_back((uint32_t)p.gates.size());
+        }
+        check(sq_run_fused(e_, order.data(), sizes.data(), sizes.size()), "sq_run_fused");
+        return;
+    }
     check(sq_apply_gates(e_, circuit.data(), circuit.size()), "sq_apply_gates");
 }
 
@@ -284,6 +300,7 @@ void HBM_Runner::run(std::vector<std::vector<sq_gate>> &subcircuits) {
 
 Stream_Runner::Stream_Runner(const Segments &seg, const Env &env) {
     env_ = env;
+    streamed_ = true;
     n_ = seg.N;
     sq_config cfg{};
     cfg.total_qbit = seg.N;

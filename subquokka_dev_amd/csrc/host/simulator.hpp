@@ -116,6 +116,7 @@ protected:
     sq_engine *e_ = nullptr;
     Env env_;
     int n_ = 0;
+    bool streamed_ = false;   // out-of-core runner: fused blocks need the state in HBM
 };
 
 class Stream_Runner : public HBM_Runner {

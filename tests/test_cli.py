@@ -46,6 +46,7 @@ def test_missing_ini(tmp_path):
     (dict(total_qbit=6, chunk_qbit=2, runner_type="RDMA"), "RDMA runner not found"),
     (dict(total_qbit=6, chunk_qbit=2, mpi_qbit=1, runner_type="MPI_MEM"), "mpi_qbit"),
     (dict(total_qbit=6, chunk_qbit=2, runner_type="MEM", precision="c32"), "precision"),
+    (dict(total_qbit=6, chunk_qbit=2, runner_type="MEM", fused_k=7), "fused_k"),
 ])
 def test_config_errors(tmp_path, kv, msg):
     write_ini(tmp_path / "a.ini", **kv)
@@ -110,6 +111,21 @@ def test_cli_c64_precision_key(tmp_path, oracle):
     assert r.returncode == 0, r.stderr
     want = oracle.run(sq.gen_state(n, 3), n, text)
     assert np.max(np.abs(dumped(tmp_path / "out.bin", n) - want)) < 2e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("c128", 1e-12), ("c64", 2e-6)])
+def test_cli_fused_k_blocks(tmp_path, oracle, precision, tol):
+    # fused_k=5: the plain circuit is planned into 5-qubit blocks on MFMA
+    n = 13
+    text = random_circuit(n, 150, seed=12)
+    open(tmp_path / "c.txt", "w").write(text)
+    write_ini(tmp_path / "a.ini", total_qbit=n, chunk_qbit=4, runner_type="MEM", is_subcircuit=0,
+              precision=precision, fused_k=5, init_seed=4, dump_binary="out.bin")
+    r = run(["-i", "a.ini", "-c", "c.txt"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    want = oracle.run(sq.gen_state(n, 4), n, text)
+    assert np.max(np.abs(dumped(tmp_path / "out.bin", n) - want)) < tol
 
 
 @pytest.mark.gpu
