@@ -3,16 +3,20 @@
 
 Workload: the 30-qubit random-layer circuit (20 layers; each layer U3(theta,
 phi, lambda) on every qubit, emitted as U1, plus CZ on a seeded perfect
-matching, emitted as CP pi; 900 gates, seed 2) applied gate by gate to a
+matching, emitted as CP pi; 900 gates, seed 2) applied as a plain gate list
+(sq_apply_gates, the MEM_Runner::run(vector<Gate*>&) interface) to a
 complex128 state in HBM (16 GiB) from |0...0>.  One step = the whole 900-gate
-circuit.  The same circuit is then run on a complex64 state (8 GiB) and its
-infidelity against the complex128 result is reported.
+circuit, planning included.  The engine runs the list as register passes (a
+run of one-qubit and diagonal gates per read and write of the state, each gate
+with its own 2x2 or diagonal arithmetic); the same list one launch per gate is
+reported beside it.  The fused planner blocks (complex64 and complex128) and
+a complex64 state are reported against the complex128 result.
 
 Prints one JSON line (rank 0).  value = complex128 gates/s over the K timed
-steps; roofline = the dominant kernel (dense one-qubit sweep), timed with HIP
-events on the engine's stream during the timed steps; cpu_baseline = the
-reference's own MEM gate code (oracle/_ref) or the oracle port, on this host,
-on a bounded sample of the same circuit (its first layer, 45 gates at 30 qubits).
+steps; roofline = the kernel with the most time in those steps, timed with HIP
+events on the engine's stream; cpu_baseline = the reference's own MEM gate
+code (oracle/_ref) or the oracle port, on this host, on a bounded sample of
+the same circuit (its first layer, 45 gates at 30 qubits).
 """
 import argparse
 import json
@@ -27,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "30q random circuit: gates/s, HBM GB/s; fused c64 vs plain c128 @1e-6 infid"
 HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X vector FP64 (no packed FP32 in these kernels)
 N_QUBITS, LAYERS, SEED = 30, 20, 2
 
 
@@ -289,9 +294,11 @@ def workload_qft20(args):
 def workload_traffic(args):
     """A short fixed kernel sequence whose HBM bytes per dispatch rocprofv3
     --pmc reads (FETCH_SIZE, WRITE_SIZE): a 4 GiB device copy of known size
-    (the scale factor), then on a 30-qubit complex128 state 3 dense one-qubit
-    gates with the target at or above bit 6, 3 with the target in bits 0-5,
-    3 CP gates, and one fused 5-qubit block in complex128 and in complex64."""
+    (the scale factor), then on a 30-qubit complex128 state, one launch per
+    gate, 3 dense one-qubit gates with the target at or above bit 6, 3 with
+    the target in bits 0-5 and 3 CP gates; the first two layers of the
+    30-qubit random circuit as register passes; one fused 5-qubit block in
+    complex128 and in complex64."""
     import torch
     import subquokka_dev_amd as sq
     n = 30
@@ -311,9 +318,16 @@ def workload_traffic(args):
     u = "0.6 0.0 0.0 0.8 0.8 0.0 0.0 -0.6"
     text = "".join(f"U1 {q} {u}\n" for q in (20, 25, 29)) + "".join(f"U1 {q} {u}\n" for q in (0, 2, 5)) + \
         "".join(f"CP {a} {b} 3.141592653589793\n" for a, b in ((3, 17), (8, 22), (11, 28)))
-    eng = sq.Engine(n, dtype="c128")
+    eng = sq.Engine(n, dtype="c128", sweep_mode=1)
     eng.init_zero()
     eng.apply_text(text)
+    eng.sync()
+    eng.close()
+    # register passes: the first two layers of the 30-qubit random circuit
+    layers2 = "\n".join(sq.gen_circuit("random", n, 20, seed=SEED).splitlines()[:90]) + "\n"
+    eng = sq.Engine(n, dtype="c128")
+    eng.init_zero()
+    eng.apply_text(layers2)
     eng.sync()
     eng.close()
     blk = "".join(f"U1 {q} {u}\n" for q in (6, 12, 18, 24, 29)) + "CP 6 29 0.5\n"
@@ -349,8 +363,8 @@ def main():
     gates, _, ng, _ = sq.parse_circuit(text, n)
     print(f"[bench] {n} qubits, {layers} layers, {ng} gates", file=sys.stderr, flush=True)
 
-    def run_steps(dtype, steps, warmup, stats):
-        eng = sq.Engine(n, dtype=dtype)
+    def run_steps(dtype, steps, warmup, stats, sweep_mode=0):
+        eng = sq.Engine(n, dtype=dtype, sweep_mode=sweep_mode)
         if dtype == "c128":
             out_extra["hbm_copy_gbs"] = copy_ceiling(eng, n)
         for _ in range(warmup):
@@ -367,7 +381,8 @@ def main():
             eng.apply_gates(gates, ng)
             eng.sync()
             times.append(time.perf_counter() - t0)
-            print(f"[bench] {dtype} step {len(times)}: {times[-1]*1e3:.1f} ms", file=sys.stderr, flush=True)
+            print(f"[bench] {dtype} sweep_mode {sweep_mode} step {len(times)}: {times[-1]*1e3:.1f} ms",
+                  file=sys.stderr, flush=True)
         st = eng.stats() if stats else {}
         eng.stats_enable(False)
         return eng, times, st
@@ -388,6 +403,11 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom_name),
                 "kernel": dom_name, "avg_launch_ms": round(avg_ms, 4),
                 "alg_bytes_per_launch": bytes_per_launch}
+    if dom.get("alg_flops"):
+        tf = dom["alg_flops"] / dom["launches"] / (avg_ms * 1e-3) / 1e12
+        roofline["valu_fp64"] = {"achieved_tflops": round(tf, 2), "peak_tflops": FP64_VALU_PEAK_TFLOPS,
+                                 "frac": round(tf / FP64_VALU_PEAK_TFLOPS, 4),
+                                 "alg_flops_per_launch": dom["alg_flops"] / dom["launches"]}
     kernels = {k: {"launches": v["launches"], "avg_ms": round(v["total_ms"] / v["launches"], 4),
                    "gbs": round(v["alg_bytes"] / (v["total_ms"] * 1e-3) / 1e9, 1)} for k, v in st128.items()}
 
@@ -397,12 +417,27 @@ def main():
            "data": "synthetic (seeded random-layer circuit, |0...0> initial state)",
            "config": {"workload": f"random_layers_{n}q_{layers}l_seed{SEED}_plain_c128", "n_qubits": n,
                       "layers": layers, "gates": ng, "state": "complex128 in HBM",
-                      "mode": "plain gate-by-gate (MEM_Runner::run(vector<Gate*>&) form)"},
+                      "mode": "plain gate list (MEM_Runner::run(vector<Gate*>&) form) as register passes, "
+                              "4 register qubits + 6 lane qubits per pass"},
            "hbm_gbs": round(hbm_gbs, 1), "roofline": roofline, "kernels_c128": kernels}
     if out_extra.get("hbm_copy_gbs"):
         out["hbm_copy_ceiling"] = {"gbs": round(out_extra["hbm_copy_gbs"], 1),
                                    "what": "hipMemcpyDtoD of 8 GiB inside the state buffer (read + write bytes)",
                                    "dominant_kernel_over_copy": round(achieved / out_extra["hbm_copy_gbs"], 4)}
+
+    # the same gate list, one launch per gate (the reference's one pass per gate)
+    engpg, tpg, stpg = run_steps("c128", 1, 0, True, sweep_mode=1)
+    dpg = max(stpg.items(), key=lambda kv: kv[1]["total_ms"])
+    out["plain_one_launch_per_gate_c128"] = {
+        "gates_s": round(ng / sum(tpg), 2), "ms_per_step": round(sum(tpg) * 1e3, 2),
+        "register_pass_speedup": round(sum(tpg) / (total_t / args.steps), 2),
+        "max_abs_diff_vs_register_passes": eng128.max_abs_diff(engpg),
+        "dominant_kernel": {"name": dpg[0], "avg_ms": round(dpg[1]["total_ms"] / dpg[1]["launches"], 4),
+                            "gbs": round(dpg[1]["alg_bytes"] / (dpg[1]["total_ms"] * 1e-3) / 1e9, 1),
+                            "hbm_frac": round(dpg[1]["alg_bytes"] / (dpg[1]["total_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernels": {k: {"launches": v["launches"], "avg_ms": round(v["total_ms"] / v["launches"], 4),
+                        "gbs": round(v["alg_bytes"] / (v["total_ms"] * 1e-3) / 1e9, 1)} for k, v in stpg.items()}}
+    engpg.close()
 
     if not args.no_fused:
         # Planner (host, once per circuit, outside the timed region -- the

@@ -90,7 +90,11 @@ typedef struct sq_config {
     int32_t resident_qbit;    /* 0: whole state in HBM (MEM runner).  >0: out-of-core
                                  runner, state in pinned host memory, 2^resident_qbit
                                  amplitudes per streamed chunk (DIO_Runner role) */
-    int32_t reserved[3];
+    int32_t sweep_mode;       /* plain sweep: 0 = register passes (a run of one-qubit
+                                 and diagonal gates per read/write of the state),
+                                 1 = one launch per gate */
+    int32_t pass_qbits;       /* register qubits per pass, 1..4 (0: 4) */
+    int32_t reserved[1];
 } sq_config;
 
 typedef struct sq_engine sq_engine;
@@ -156,6 +160,7 @@ typedef struct sq_kstat {
     uint64_t launches;
     double total_ms;
     double alg_bytes;   /* algorithmic bytes: 2*B*2^N*f_g summed over launches */
+    double alg_flops;   /* algorithmic real flops (0 where not counted) */
 } sq_kstat;
 int sq_stats_enable(sq_engine *e, int32_t on);
 int sq_stats_get(sq_engine *e, sq_kstat *out, int32_t cap, int32_t *n_out);
@@ -188,9 +193,14 @@ int sq_gen_circuit(int32_t kind, int32_t n_qubits, int32_t layers, uint64_t seed
 /* Gaussian (re, im) i.i.d., normalised in FP64: 2^n complex<double>. */
 int sq_gen_state(int32_t n_qubits, uint64_t seed, double *out);
 
-/* Planner: groups a plain circuit into blocks.  mode 0 = subcircuit text with
- * VSWAP passes for a chunk of chunk_qbit low qubits (finder.cpp:69-333 format);
- * mode 1 = fused k-qubit blocks (fused_k).  Output text into buf. */
+/* Planner: groups a plain circuit into blocks (finder role, finder.cpp:629-877).
+ * Output is subcircuit text ("<count>" then count gate lines, repeated) in buf.
+ *   mode 0: blocks with relabelled targets and VSWAP passes for a chunk of
+ *           param low qubits (finder.cpp:69-333 format);
+ *   mode 3: mode 0 plus swaps that restore the qubit order at the end;
+ *   mode 1: fused blocks of at most param qubits (fused_k);
+ *   mode 2: the plain sweep's register-pass schedule for complex128 (qubits
+ *           0-5 resident plus at most param register qubits per step). */
 int sq_plan(const char *circuit_text, int32_t total_qbit, int32_t mode, int32_t param,
             char *buf, uint64_t cap, uint64_t *len);
 

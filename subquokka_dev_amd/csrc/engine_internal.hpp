@@ -12,6 +12,7 @@
 struct PendingStat {
     char name[48];
     double alg_bytes;
+    double alg_flops;
     hipEvent_t start, stop;
 };
 
@@ -43,6 +44,10 @@ struct sq_engine {
     void *d_gradpart = nullptr;     // adjoint: per-parameter block partials
     size_t gradpart_cap = 0;
     uint64_t n_tile_passes = 0;     // tile passes launched (diagnostics)
+    void *d_passprog = nullptr;     // register-pass program of the plain sweep
+    size_t passprog_cap = 0;
+    int sweep_mode = 0;             // 0: register passes, 1: one launch per gate
+    int pass_qbits = 4;             // register qubits per pass (1..5)
 
     // out-of-core runner (resident_qbit > 0): state in pinned host memory
     int res_qbit = 0;
@@ -70,11 +75,12 @@ struct sq_engine {
     int check_launch(const char *what);
     unsigned grid_for(uint64_t work_items, unsigned per_block) const;
     unsigned grid_flat(uint64_t work_items, unsigned per_block) const;
-    int stat_begin(const char *name, double alg_bytes);
+    int stat_begin(const char *name, double alg_bytes, double alg_flops = 0.0);
     void stat_end(int slot);
 
     int lower(const sq_gate &g, sq::LoweredGate &lg);
     int apply_lowered(const sq::LoweredGate &g);
+    int apply_plain(const std::vector<sq_gate> &gates, const std::vector<sq::LoweredGate> &low);
     int ensure_partials();
     int norm_zz(double *v);
     int compare(sq_engine *o, double *v);

@@ -132,6 +132,89 @@ std::vector<PlanPass> plan_passes(const std::vector<sq_gate> &gates, int n, int 
     return passes;
 }
 
+namespace {
+
+// Gates of `pending` that a register pass with resident qubit set `in` can
+// run, in order.  Stops once every qubit is blocked.
+int count_pass(const std::vector<sq_gate> &gates, const std::vector<int> &pending, const std::vector<char> &in,
+               const std::vector<char> &mergeable, const std::vector<char> &anywhere, std::vector<char> &blocked,
+               std::vector<int> *out) {
+    std::fill(blocked.begin(), blocked.end(), 0);
+    const int n = (int)blocked.size();
+    int cnt = 0, n_blocked = 0;
+    for (int gi : pending) {
+        const sq_gate &g = gates[gi];
+        bool ok = mergeable[gi] != 0;
+        for (int i = 0; ok && i < g.nq; i++)
+            if (blocked[g.q[i]] || (!anywhere[gi] && !in[g.q[i]])) ok = false;
+        if (ok) {
+            cnt++;
+            if (out) out->push_back(gi);
+            continue;
+        }
+        for (int i = 0; i < g.nq; i++)
+            if (!blocked[g.q[i]]) { blocked[g.q[i]] = 1; n_blocked++; }
+        if (n_blocked == n) break;
+    }
+    return cnt;
+}
+
+}  // namespace
+
+std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, const std::vector<char> &mergeable,
+                                            const std::vector<char> &anywhere, int n, int low, int kh) {
+    std::vector<SweepStep> steps;
+    std::vector<int> pending(gates.size());
+    for (size_t i = 0; i < gates.size(); i++) pending[i] = (int)i;
+    std::vector<char> in(n), blocked(n);
+    while (!pending.empty()) {
+        const int g0 = pending.front();
+        if (!mergeable[g0]) {
+            SweepStep st;
+            st.gates.push_back(g0);
+            steps.push_back(std::move(st));
+            pending.erase(pending.begin());
+            continue;
+        }
+        std::fill(in.begin(), in.end(), 0);
+        for (int q = 0; q < std::min(low, n); q++) in[q] = 1;
+        std::vector<int> regs;
+        int best = count_pass(gates, pending, in, mergeable, anywhere, blocked, nullptr);
+        while ((int)regs.size() < kh) {
+            int best_q = -1, best_c = best;
+            for (int q = low; q < n; q++) {
+                if (in[q]) continue;
+                in[q] = 1;
+                const int c = count_pass(gates, pending, in, mergeable, anywhere, blocked, nullptr);
+                in[q] = 0;
+                if (c > best_c) { best_c = c; best_q = q; }
+            }
+            if (best_q < 0) break;
+            in[best_q] = 1;
+            regs.push_back(best_q);
+            best = best_c;
+        }
+        SweepStep st;
+        st.merged = true;
+        count_pass(gates, pending, in, mergeable, anywhere, blocked, &st.gates);
+        if (st.gates.empty()) {   // cannot happen for a mergeable first gate; stay safe
+            st.merged = false;
+            st.gates.push_back(g0);
+        }
+        std::sort(regs.begin(), regs.end());
+        st.regs = regs;
+        std::vector<int> rest;
+        size_t k = 0;
+        for (int gi : pending) {
+            if (k < st.gates.size() && st.gates[k] == gi) { k++; continue; }
+            rest.push_back(gi);
+        }
+        pending.swap(rest);
+        steps.push_back(std::move(st));
+    }
+    return steps;
+}
+
 std::string plan_subcircuit_text(const std::vector<sq_gate> &gates, int n, int chunk, bool restore) {
     std::vector<int> pos(n), at(n);
     for (int q = 0; q < n; q++) pos[q] = at[q] = q;
@@ -234,6 +317,21 @@ extern "C" int sq_plan(const char *circuit_text, int32_t n, int32_t mode, int32_
         for (auto &p : sq::plan_passes(gates, n, param, {})) {
             out += std::to_string(p.gates.size()) + "\n";
             for (int gi : p.gates) out += sq::gate_to_line(gates[gi]) + "\n";
+        }
+    } else if (mode == 2) {
+        // register passes of the complex128 plain sweep: lane qubits 0-5 plus
+        // at most param register qubits; other gates stand alone
+        if (param < 1 || param > 4) return SQ_ERR_ARG;
+        std::vector<char> mergeable(gates.size(), 0), anywhere(gates.size(), 0);
+        for (size_t i = 0; i < gates.size(); i++) {
+            sq::LoweredGate lg;
+            if (sq::lower_gate(gates[i], n, lg, nullptr) != SQ_OK) return SQ_ERR_ARG;
+            if (lg.cls == sq::OpClass::Diag && lg.k <= 2) mergeable[i] = anywhere[i] = 1;
+            else if (lg.cls == sq::OpClass::Dense && lg.k == 1) mergeable[i] = 1;
+        }
+        for (auto &st : sq::plan_register_passes(gates, mergeable, anywhere, n, 6, param)) {
+            out += std::to_string(st.gates.size()) + "\n";
+            for (int gi : st.gates) out += sq::gate_to_line(gates[gi]) + "\n";
         }
     } else {
         return SQ_ERR_ARG;

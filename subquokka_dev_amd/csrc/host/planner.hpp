@@ -27,6 +27,22 @@ std::vector<int> gate_qubits(const sq_gate &g);
 std::vector<PlanPass> plan_passes(const std::vector<sq_gate> &gates, int n, int budget,
                                   const std::vector<int> &fixed);
 
+// Plain-sweep schedule of register passes.  A merged step applies its gates
+// in one read and one write of the state with the qubits [0, low) plus at
+// most kh register qubits resident; any other step is one gate on its own.
+//   mergeable[i]: gate i may run inside a register pass;
+//   anywhere[i]:  gate i may run whatever the pass's qubit set (diagonal
+//                 gates: the phase needs only the amplitude's index).
+// Per-qubit order is kept: a gate joins a step only after every earlier
+// gate sharing a qubit with it.
+struct SweepStep {
+    bool merged = false;
+    std::vector<int> gates;      // indices into the input, in execution order
+    std::vector<int> regs;       // register qubits of a merged step, ascending
+};
+std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, const std::vector<char> &mergeable,
+                                            const std::vector<char> &anywhere, int n, int low, int kh);
+
 // Reference subcircuit text (finder.cpp:69-333 format) for a chunk of
 // chunk_qbit low qubits: gate blocks with relabelled targets separated by
 // VSWAP_1_1 / VSWAP_2_2 passes.  restore=true appends swaps that return every

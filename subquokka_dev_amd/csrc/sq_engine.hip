@@ -48,11 +48,12 @@ int sq_engine::hip_fail(hipError_t e, const char *what) {
 // launch bookkeeping: optional HIP events around each launch
 // ---------------------------------------------------------------------------
 
-int sq_engine::stat_begin(const char *name, double alg_bytes) {
+int sq_engine::stat_begin(const char *name, double alg_bytes, double alg_flops) {
     if (!stats_on) return -1;
     PendingStat ps;
     std::snprintf(ps.name, sizeof ps.name, "%s", name);
     ps.alg_bytes = alg_bytes;
+    ps.alg_flops = alg_flops;
     if (hipEventCreate(&ps.start) != hipSuccess || hipEventCreate(&ps.stop) != hipSuccess) return -1;
     hipEventRecord(ps.start, stream);
     pending.push_back(ps);
@@ -273,6 +274,7 @@ int sq_create(const sq_config *cfg, sq_engine **out) {
     if (cfg->total_qbit < 1 || cfg->total_qbit > 36) return SQ_ERR_ARG;
     if (cfg->dtype != SQ_C128 && cfg->dtype != SQ_C64) return SQ_ERR_ARG;
     if (cfg->chunk_qbit < 0 || cfg->chunk_qbit > cfg->total_qbit) return SQ_ERR_ARG;
+    if (cfg->sweep_mode < 0 || cfg->sweep_mode > 1 || cfg->pass_qbits < 0 || cfg->pass_qbits > 4) return SQ_ERR_ARG;
     // a resident chunk at least as large as the state means the state fits in HBM
     const int resident = cfg->resident_qbit >= cfg->total_qbit ? 0 : cfg->resident_qbit;
     if (resident < 0) return SQ_ERR_ARG;
@@ -283,6 +285,8 @@ int sq_create(const sq_config *cfg, sq_engine **out) {
     e->chunk = cfg->chunk_qbit > 0 ? cfg->chunk_qbit : std::min(cfg->total_qbit, 12);
     e->dtype = cfg->dtype;
     e->device = cfg->device;
+    e->sweep_mode = cfg->sweep_mode;
+    if (cfg->pass_qbits > 0) e->pass_qbits = cfg->pass_qbits;
     if (hipSetDevice(e->device) != hipSuccess) { delete e; return SQ_ERR_HIP; }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { delete e; return SQ_ERR_HIP; }
     if (resident != 0) {
@@ -418,11 +422,7 @@ int sq_apply_gates(sq_engine *e, const sq_gate *gates, uint64_t n) {
         if (rc) { e->err += " (gate " + std::to_string(i) + ")"; return rc; }
     }
     if (e->out_of_core()) return e->stream_apply(std::vector<sq_gate>(gates, gates + n), low);
-    for (uint64_t i = 0; i < n; i++) {
-        int rc = e->apply_lowered(low[i]);
-        if (rc) return rc;
-    }
-    return SQ_OK;
+    return e->apply_plain(std::vector<sq_gate>(gates, gates + n), low);
 }
 
 int sq_norm2(sq_engine *e, double *out) {
@@ -500,6 +500,7 @@ int sq_stats_get(sq_engine *e, sq_kstat *out, int32_t cap, int32_t *n_out) {
         it->launches++;
         it->total_ms += ms;
         it->alg_bytes += p.alg_bytes;
+        it->alg_flops += p.alg_flops;
     }
     if (n_out) *n_out = (int32_t)agg.size();
     for (int i = 0; i < (int)agg.size() && i < cap; i++) out[i] = agg[i];

@@ -1,0 +1,184 @@
+// sq_pass.hip -- the plain sweep (MEM_Runner::run(vector<Gate*>&),
+// MEM_Runner.cpp:101-134) as a schedule of register passes.
+//
+// The planner (plan_register_passes) cuts the gate list into steps: a merged
+// step is a run of one-qubit dense and diagonal gates whose dense targets lie
+// in the lane bits or in at most pass_qbits chosen register qubits; it costs
+// one read and one write of the state (kernels/pass.hpp).  Every other gate
+// (two-qubit dense, permutations) runs alone with its own kernel, and so does
+// a merged step that would cost no more than one pass done gate by gate.
+// Per-qubit gate order is kept, so the schedule computes the same map as the
+// gates in input order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "subquokka_hip.h"
+#include "engine_internal.hpp"
+#include "host/planner.hpp"
+#include "kernels/pass.hpp"
+
+using sq::LoweredGate;
+using sq::OpClass;
+using namespace sqk;
+
+namespace {
+
+struct PassLaunch {
+    bool merged = false;
+    int gate = -1;                 // single-gate step
+    int op_begin = 0, n_ops = 0;   // merged step
+    double flops_per_amp = 0;      // algorithmic real flops per amplitude
+    int hpos[kPassMaxKH] = {0, 0, 0, 0};
+};
+
+template <typename T>
+void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vector<PassOp> &ops,
+                    std::vector<cx<T>> &coef) {
+    constexpr int VB = Traits<T>::VB;
+    constexpr int S = VB + 6;
+    PassOp o{};
+    o.coef = (uint32_t)coef.size();
+    const int nc = g.cls == OpClass::Dense ? 4 : (1 << g.k);
+    for (int i = 0; i < nc; i++) coef.push_back({(T)g.m[i].real(), (T)g.m[i].imag()});
+    if (g.cls == OpClass::Dense) {
+        const int t = g.t[0];
+        if (t < VB) {
+            o.kind = PO_DENSE_VEC;
+        } else if (t < S) {
+            o.kind = PO_DENSE_LANE;
+            o.a = (uint8_t)(t - VB);
+        } else {
+            o.kind = PO_DENSE_REG;
+            o.a = (uint8_t)(std::find(regs.begin(), regs.end(), t) - regs.begin());
+        }
+    } else {
+        o.kind = g.k == 1 ? PO_DIAG1 : PO_DIAG2;
+        o.a = (uint8_t)g.t[0];
+        o.b = (uint8_t)(g.k == 2 ? g.t[1] : 0);
+    }
+    ops.push_back(o);
+}
+
+template <typename T, int KH>
+int launch_pass(sq_engine *e, const PassArgs<T> &a) {
+    const unsigned grid = e->grid_flat(a.n_tiles, kBlock / 64);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pass<T, KH>), grid, kBlock, 0, e->stream,
+                       reinterpret_cast<cx<T> *>(e->d_state), a);
+    return e->check_launch("k_pass");
+}
+
+template <typename T>
+int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low) {
+    constexpr int VB = Traits<T>::VB;
+    constexpr int S = VB + 6;
+    const int n = e->n;
+    const int kh = e->pass_qbits;
+    if (e->sweep_mode == 1 || n < S + kh) {
+        for (const auto &g : low) {
+            int rc = e->apply_lowered(g);
+            if (rc) return rc;
+        }
+        return SQ_OK;
+    }
+    const size_t ng = gates.size();
+    std::vector<char> mergeable(ng, 0), anywhere(ng, 0);
+    for (size_t i = 0; i < ng; i++) {
+        const LoweredGate &g = low[i];
+        if (g.cls == OpClass::Diag && g.k <= 2) mergeable[i] = anywhere[i] = 1;
+        else if (g.cls == OpClass::Dense && g.k == 1) mergeable[i] = 1;
+    }
+    const auto steps = sq::plan_register_passes(gates, mergeable, anywhere, n, S, kh);
+
+    std::vector<PassLaunch> launches;
+    std::vector<PassOp> ops;
+    std::vector<cx<T>> coef;
+    for (const auto &st : steps) {
+        double frac = 0;
+        for (int gi : st.gates) frac += low[gi].pass_fraction;
+        if (!st.merged || frac <= 1.0) {
+            for (int gi : st.gates) {
+                PassLaunch l;
+                l.gate = gi;
+                launches.push_back(l);
+            }
+            continue;
+        }
+        // exactly kh register qubits: pad with qubits the pass does not touch
+        std::vector<int> regs = st.regs;
+        for (int q = S; q < n && (int)regs.size() < kh; q++)
+            if (std::find(regs.begin(), regs.end(), q) == regs.end()) regs.push_back(q);
+        std::sort(regs.begin(), regs.end());
+        PassLaunch l;
+        l.merged = true;
+        l.op_begin = (int)ops.size();
+        for (int gi : st.gates) {
+            encode_pass_op<T>(low[gi], regs, ops, coef);
+            // dense 2x2 row: two complex products and a sum; diagonal: one product
+            l.flops_per_amp += low[gi].cls == OpClass::Dense ? 14.0 : 6.0;
+        }
+        l.n_ops = (int)ops.size() - l.op_begin;
+        for (int j = 0; j < kh; j++) l.hpos[j] = regs[j];
+        launches.push_back(l);
+    }
+
+    // upload the program once; earlier launches may still read the old one
+    const size_t ops_bytes = ((ops.size() * sizeof(PassOp) + 255) / 256) * 256;
+    const size_t need = ops_bytes + coef.size() * sizeof(cx<T>) + 256;
+    if (!ops.empty()) {
+        if (hipStreamSynchronize(e->stream) != hipSuccess) return e->fail(SQ_ERR_HIP, "stream sync");
+        if (need > e->passprog_cap) {
+            if (e->d_passprog) hipFree(e->d_passprog);
+            e->d_passprog = nullptr;
+            e->passprog_cap = 0;
+            hipError_t he = hipMalloc(&e->d_passprog, need * 2);
+            if (he != hipSuccess) return e->hip_fail(he, "hipMalloc(pass program)");
+            e->passprog_cap = need * 2;
+        }
+        char *d = static_cast<char *>(e->d_passprog);
+        hipError_t he = hipMemcpy(d, ops.data(), ops.size() * sizeof(PassOp), hipMemcpyHostToDevice);
+        if (he == hipSuccess)
+            he = hipMemcpy(d + ops_bytes, coef.data(), coef.size() * sizeof(cx<T>), hipMemcpyHostToDevice);
+        if (he != hipSuccess) return e->hip_fail(he, "hipMemcpy(pass program)");
+    }
+    const PassOp *dops = reinterpret_cast<const PassOp *>(e->d_passprog);
+    const cx<T> *dcoef = reinterpret_cast<const cx<T> *>(static_cast<char *>(e->d_passprog) + ops_bytes);
+
+    char nm[48];
+    std::snprintf(nm, sizeof nm, "pass%d_%s", kh, e->dtype == SQ_C128 ? "c128" : "c64");
+    const double bytes = 2.0 * e->amp_bytes() * std::ldexp(1.0, n);
+    for (const auto &l : launches) {
+        if (!l.merged) {
+            int rc = e->apply_lowered(low[l.gate]);
+            if (rc) return rc;
+            continue;
+        }
+        PassArgs<T> a{};
+        a.ops = dops + l.op_begin;
+        a.coef = dcoef;
+        a.n_ops = l.n_ops;
+        a.n_tiles = 1ULL << (n - S - kh);
+        for (int j = 0; j < kh; j++) a.hpos[j] = l.hpos[j];
+        int slot = e->stat_begin(nm, bytes, l.flops_per_amp * std::ldexp(1.0, n));
+        int rc;
+        switch (kh) {
+        case 1: rc = launch_pass<T, 1>(e, a); break;
+        case 2: rc = launch_pass<T, 2>(e, a); break;
+        case 3: rc = launch_pass<T, 3>(e, a); break;
+        case 4: rc = launch_pass<T, 4>(e, a); break;
+        default: rc = e->fail(SQ_ERR_ARG, "internal: bad pass width");
+        }
+        e->stat_end(slot);
+        if (rc) return rc;
+    }
+    return SQ_OK;
+}
+
+}  // namespace
+
+int sq_engine::apply_plain(const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low) {
+    return dtype == SQ_C128 ? run_plain<double>(this, gates, low) : run_plain<float>(this, gates, low);
+}

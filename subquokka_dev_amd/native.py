@@ -41,12 +41,13 @@ class SqFused(ctypes.Structure):
 
 class SqConfig(ctypes.Structure):
     _fields_ = [("total_qbit", ctypes.c_int32), ("chunk_qbit", ctypes.c_int32), ("dtype", ctypes.c_int32),
-                ("device", ctypes.c_int32), ("resident_qbit", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("device", ctypes.c_int32), ("resident_qbit", ctypes.c_int32), ("sweep_mode", ctypes.c_int32),
+                ("pass_qbits", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
 
 
 class SqKstat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double),
-                ("alg_bytes", ctypes.c_double)]
+                ("alg_bytes", ctypes.c_double), ("alg_flops", ctypes.c_double)]
 
 
 class SqError(RuntimeError):
@@ -193,9 +194,11 @@ def plan(text, n, mode, param):
 class Engine:
     """One state on one GPU (owns device memory until close())."""
 
-    def __init__(self, n, dtype="c128", chunk=0, device=0, resident_qbit=0):
+    def __init__(self, n, dtype="c128", chunk=0, device=0, resident_qbit=0, sweep_mode=0, pass_qbits=0):
+        """sweep_mode 0: plain sweeps run as register passes (pass_qbits register
+        qubits, 0 = default); 1: one launch per gate."""
         cfg = SqConfig(total_qbit=n, chunk_qbit=chunk, dtype=SQ_C128 if dtype == "c128" else SQ_C64,
-                       device=device, resident_qbit=resident_qbit)
+                       device=device, resident_qbit=resident_qbit, sweep_mode=sweep_mode, pass_qbits=pass_qbits)
         h = ctypes.c_void_p()
         rc = lib().sq_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != SQ_OK:
@@ -313,7 +316,8 @@ class Engine:
         n = ctypes.c_int32()
         self._c(lib().sq_stats_get(self.h, arr, 64, ctypes.byref(n)))
         return {arr[i].name.decode(): dict(launches=arr[i].launches, total_ms=arr[i].total_ms,
-                                           alg_bytes=arr[i].alg_bytes) for i in range(n.value)}
+                                           alg_bytes=arr[i].alg_bytes, alg_flops=arr[i].alg_flops)
+                for i in range(n.value)}
 
     def hea_energy(self, layers, theta):
         th = np.ascontiguousarray(theta, dtype=np.float64)

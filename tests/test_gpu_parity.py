@@ -131,6 +131,40 @@ def test_random_circuit(oracle, dtype, n):
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("kh", [1, 2, 3, 4])
+def test_register_passes_match_one_launch_per_gate(oracle, dtype, kh):
+    # the plain sweep as register passes (kh register qubits) against the same
+    # gates one launch each, and both against the oracle
+    n = 16
+    text = sq.gen_circuit("random", n, 6, seed=kh) + random_circuit(n, 200, seed=10 + kh)
+    s0 = random_state(n, kh)
+    say(f"register passes n={n} kh={kh} {dtype}")
+    host = s0.astype(np.complex128 if dtype == "c128" else np.complex64)
+    a = sq.Engine(n, dtype=dtype, pass_qbits=kh)
+    b = sq.Engine(n, dtype=dtype, sweep_mode=1)
+    try:
+        a.set_state(host)
+        b.set_state(host)
+        a.apply_text(text)
+        b.apply_text(text)
+        assert a.max_abs_diff(b) <= (1e-14 if dtype == "c128" else 1e-6)
+        check(a.get_state(np.complex128), oracle.run(s0, n, text), dtype, f"passes kh={kh}")
+    finally:
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [10, 11, 12])
+def test_register_passes_smallest_states(oracle, dtype, n):
+    # n = lane bits + register qubits exactly (one wave tile) and just above
+    text = random_circuit(n, 250, seed=3 * n) + sq.gen_circuit("random", n, 3, seed=n)
+    s0 = random_state(n, n)
+    say(f"register passes small n={n} {dtype}")
+    check(run_gpu(n, text, s0, dtype), oracle.run(s0, n, text), dtype, f"small passes n={n}")
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
 def test_config1_qft20_seeded_state(oracle, dtype):
     # 20-qubit QFT in cir_qft.py order on the seeded Gaussian state, one MEM chunk.
     n = 20

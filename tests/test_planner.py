@@ -91,3 +91,53 @@ def test_fused_plan_on_random_layers_reduces_passes():
 def test_plan_rejects_bad_params():
     with pytest.raises(sq.SqError):
         sq.plan("H 0\n", 4, 1, 9)
+
+
+DIAG = {"Z", "P", "RZ", "CP", "RZZ"}
+DENSE1 = {"H", "X", "Y", "U1", "RX", "RY"}
+
+
+@pytest.mark.parametrize("n,kh,seed", [(10, 1, 1), (11, 2, 2), (13, 4, 3), (16, 4, 4), (16, 3, 5)])
+def test_register_pass_schedule(oracle, n, kh, seed):
+    # the plain sweep's schedule (sq_plan mode 2, complex128 lanes = qubits 0-5):
+    #   * a step of several gates holds only one-qubit dense and diagonal gates,
+    #     with its dense targets in qubits 0-5 plus at most kh other qubits;
+    #   * every qubit sees its gates in the input order;
+    #   * so the oracle computes the same state from the schedule.
+    text = sq.gen_circuit("random", n, 4, seed=seed) + random_circuit(n, 200, seed=seed)
+    plan = sq.plan(text, n, 2, kh)
+    blocks = blocks_of(plan)
+    flat = [l for b in blocks for l in b]
+    src = [l for l in text.splitlines() if l.strip()]
+
+    def norm(line):   # the plan prints reals with %.17g: compare values, not text
+        tok = line.split()
+        nq = len(qubits_of(line))
+        return (tok[0], tuple(int(x) for x in tok[1:1 + nq]), tuple(float(x) for x in tok[1 + nq:]))
+
+    flat_n, src_n = [norm(l) for l in flat], [norm(l) for l in src]
+    assert sorted(flat_n) == sorted(src_n)
+    merged = 0
+    for blk in blocks:
+        if len(blk) == 1:
+            continue
+        merged += 1
+        kinds = {l.split()[0] for l in blk}
+        assert kinds <= DIAG | DENSE1, kinds
+        regs = {qubits_of(l)[0] for l in blk if l.split()[0] in DENSE1} - set(range(6))
+        assert len(regs) <= kh, (regs, kh)
+    assert merged > 0
+    for q in range(n):
+        assert [g for g in flat_n if q in g[1]] == [g for g in src_n if q in g[1]]
+    s0 = random_state(n, seed)
+    np.testing.assert_allclose(oracle.run(s0, n, "\n".join(flat) + "\n"), oracle.run(s0, n, text), rtol=0, atol=1e-12)
+
+
+def test_register_pass_schedule_cuts_passes():
+    # 30-qubit random layers: about 6 register passes per layer in place of
+    # 30 dense and 15 diagonal one-gate sweeps
+    text = sq.gen_circuit("random", 30, 20, seed=2)
+    blocks = blocks_of(sq.plan(text, 30, 2, 4))
+    steps = len(blocks)
+    assert sum(len(b) for b in blocks) == 900
+    assert steps <= 20 * 8, steps
