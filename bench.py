@@ -240,7 +240,7 @@ def workload_stream(args):
         eng.sync()
         log(f"stream part {i + 1}/{len(parts)}: {time.perf_counter() - t0:.1f} s")
     t_stream = time.perf_counter() - t0
-    passes, link_bytes = eng.stream_info()
+    passes, link_bytes, tier = eng.stream_info()
     zz_s, nrm_s = eng.expect_zz_ring(), eng.norm2()
     eng.close()
     hbm = sq.Engine(n, dtype=dtype)
@@ -257,7 +257,7 @@ def workload_stream(args):
                 ng / t_stream, "gates/s", 1, 0, t_stream * 1e3, "f64" if dtype == "c128" else "f32",
                 {"workload": f"random_layers_{n}q_{layers}l_seed{seed}_stream_{dtype}_w{args.resident_qbit}",
                  "n_qubits": n, "layers": layers, "gates": ng, "state": f"{dtype} in pinned host memory"},
-                passes=passes, host_link_gbs=round(link_bytes / t_stream / 1e9, 1),
+                passes=passes, host_tier=tier, host_link_gbs=round(link_bytes / t_stream / 1e9, 1),
                 host_link_bytes=link_bytes, in_hbm={"seconds": round(t_hbm, 3), "gates_s": round(ng / t_hbm, 2)},
                 stream_over_hbm_time=round(t_stream / t_hbm, 2),
                 zz_ring={"stream": zz_s, "hbm": zz_h, "abs_diff": abs(zz_s - zz_h)},

@@ -82,6 +82,15 @@ typedef struct sq_fused {
     const double *u;
 } sq_fused;
 
+/* Where the out-of-core runner keeps the state (resident_qbit > 0). */
+typedef enum sq_host_tier {
+    SQ_HOST_AUTO = 0,       /* pinned host memory when the whole state can be pinned,
+                               else ordinary memory + pinned staging */
+    SQ_HOST_PAGEABLE = 1,   /* ordinary host memory + pinned staging buffers */
+    SQ_HOST_PINNED = 2,     /* (reported by sq_stream_info) */
+    SQ_HOST_FILES = 3       /* state files: sq_create_file_state */
+} sq_host_tier;
+
 typedef struct sq_config {
     int32_t total_qbit;       /* [system] total_qbit (simulator.cpp:90) */
     int32_t chunk_qbit;       /* [system] chunk_qbit (simulator.cpp:94); blocked-mode tile */
@@ -94,13 +103,24 @@ typedef struct sq_config {
                                  and diagonal gates per read/write of the state),
                                  1 = one launch per gate */
     int32_t pass_qbits;       /* register qubits per pass, 1..4 (0: 4) */
-    int32_t reserved[1];
+    int32_t host_tier;        /* out-of-core runner: SQ_HOST_AUTO or SQ_HOST_PAGEABLE */
 } sq_config;
 
 typedef struct sq_engine sq_engine;
 
 /* ---- lifetime: replaces MEM_Runner::MEM_Runner (MEM_Runner.cpp:57-88) ---- */
 int sq_create(const sq_config *cfg, sq_engine **out);
+/* Out-of-core runner whose state lives in n_files files (a power of two, at
+ * most 2^total_qbit): file i holds amplitudes [i 2^N/n_files, (i+1) 2^N/n_files)
+ * as raw little-endian (re, im) of the engine's dtype -- the reference's
+ * state files (setupFiles simulator.cpp:102-129, loader_IO test_util.py:385-403).
+ * direct_io opens them O_DIRECT (runner_type DirectIO, create_file
+ * simulator.cpp:71-86; runs and files must then be multiples of 4 KiB).
+ * keep_contents = 0 creates or truncates the files and sets |0...0>;
+ * 1 opens existing files of the right size as the current state (resume).
+ * cfg->resident_qbit must be > 0. */
+int sq_create_file_state(const sq_config *cfg, const char *const *paths, int32_t n_files, int32_t direct_io,
+                         int32_t keep_contents, sq_engine **out);
 int sq_destroy(sq_engine *e);
 /* |0...0>: buffer[0] = 1 (MEM_Runner.cpp:79-81; setupStateFile simulator.cpp:639-658) */
 int sq_init_zero(sq_engine *e);
@@ -166,9 +186,10 @@ int sq_stats_enable(sq_engine *e, int32_t on);
 int sq_stats_get(sq_engine *e, sq_kstat *out, int32_t cap, int32_t *n_out);
 int sq_stats_reset(sq_engine *e);
 /* Out-of-core runner telemetry (no reference counterpart; DIO_Runner.cpp has
- * only the "<t>s" line): streamed passes since the last sq_stats_reset and the
- * bytes those passes moved over the host link (both directions). */
-int sq_stream_info(sq_engine *e, uint64_t *passes, double *host_link_bytes);
+ * only the "<t>s" line): streamed passes since the last sq_stats_reset, the
+ * bytes those passes moved over the host link (both directions) and the host
+ * tier in use (sq_host_tier; SQ_HOST_AUTO for an HBM-resident engine). */
+int sq_stream_info(sq_engine *e, uint64_t *passes, double *host_link_bytes, int32_t *host_tier);
 
 /* ---- host-side formats (no device needed) ---- */
 /* Parses one gate line of the MEM grammar (setGate_MEM, simulator.cpp:542-597;

@@ -49,9 +49,14 @@ struct sq_engine {
     int sweep_mode = 0;             // 0: register passes, 1: one launch per gate
     int pass_qbits = 4;             // register qubits per pass (1..5)
 
-    // out-of-core runner (resident_qbit > 0): state in pinned host memory
+    // out-of-core runner (resident_qbit > 0): state outside HBM
     int res_qbit = 0;
-    void *h_state = nullptr;
+    int host_tier = SQ_HOST_AUTO;   // where the state lives (sq_host_tier)
+    void *h_state = nullptr;        // PINNED / PAGEABLE tiers
+    bool h_state_pageable = false;
+    void *h_stage[2] = {nullptr, nullptr};   // pinned staging (PAGEABLE / FILES tiers)
+    std::vector<int> file_fds;      // FILES tier: file i = amplitudes [i 2^n/nf, (i+1) 2^n/nf)
+    bool file_direct = false;
     void *d_group[2] = {nullptr, nullptr};
     hipEvent_t ev_loaded[2] = {nullptr, nullptr}, ev_computed[2] = {nullptr, nullptr},
                ev_stored[2] = {nullptr, nullptr}, ev_pass = nullptr;
@@ -59,6 +64,13 @@ struct sq_engine {
     uint64_t n_stream_passes = 0;
     bool out_of_core() const { return res_qbit > 0; }
     int stream_create();
+    int stream_open_files(const char *const *paths, int n_files, bool direct, bool keep);
+    int stream_high() const;
+    bool host_read(uint64_t off, size_t bytes, void *dst);
+    bool host_write(uint64_t off, size_t bytes, const void *src);
+    bool stage_runs(bool to_stage, uint64_t base_bytes, const std::vector<uint64_t> &run_off_bytes, size_t run_bytes,
+                    char *stage);
+    template <typename F> int host_chunks(bool write_back, F fn);
     void stream_destroy();
     int stream_init_zero();
     int stream_set_state(const void *host, int32_t host_dtype);

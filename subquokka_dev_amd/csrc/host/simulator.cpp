@@ -2,6 +2,9 @@
 #include "simulator.hpp"
 #include "planner.hpp"
 
+#include <sys/stat.h>
+#include <sys/types.h>
+
 #include <algorithm>
 #include <cctype>
 #include <chrono>
@@ -307,9 +310,37 @@ Stream_Runner::Stream_Runner(const Segments &seg, const Env &env) {
     cfg.chunk_qbit = seg.chunk;
     cfg.dtype = env.precision == "c64" ? SQ_C64 : SQ_C128;
     cfg.device = env.device;
-    cfg.resident_qbit = env.resident_qbit > 0 ? env.resident_qbit : std::min(seg.N, 30);
-    int rc = sq_create(&cfg, &e_);
-    if (rc != SQ_OK) throw SimError(rc, "sq_create (out-of-core) failed");
+    if (env.state_paths.empty()) {
+        // state in host memory (no state files named)
+        cfg.resident_qbit = env.resident_qbit > 0 ? env.resident_qbit : std::min(seg.N, 30);
+        int rc = sq_create(&cfg, &e_);
+        if (rc != SQ_OK) throw SimError(rc, "sq_create (out-of-core) failed");
+        return;
+    }
+    // state files: 2^file_qbit comma-separated paths (setupFiles,
+    // simulator.cpp:102-129); their directories are created as needed
+    std::vector<std::string> paths;
+    std::stringstream ss(env.state_paths);
+    std::string tok;
+    while (std::getline(ss, tok, ',')) if (!tok.empty()) paths.push_back(tok);
+    const size_t nf = (size_t)1 << seg.file;
+    if (paths.size() < nf) throw SimError(SQ_ERR_ARG, "[Config File]: state_paths lists fewer than 2^file_qbit files");
+    paths.resize(nf);
+    for (const auto &p : paths) {
+        const size_t cut = p.find_last_of('/');
+        if (cut == std::string::npos || cut == 0) continue;
+        std::string dir;
+        for (size_t i = 0; i <= cut; i++) {
+            if (p[i] == '/' && !dir.empty()) mkdir(dir.c_str(), 0777);
+            dir += p[i];
+        }
+    }
+    std::vector<const char *> cp;
+    for (const auto &p : paths) cp.push_back(p.c_str());
+    cfg.resident_qbit = env.resident_qbit > 0 ? env.resident_qbit : std::min(seg.N - 1, 30);
+    const bool direct = env.runner_type == "DirectIO" || env.runner_type == "MPI_DIO";
+    int rc = sq_create_file_state(&cfg, cp.data(), (int32_t)cp.size(), direct ? 1 : 0, 0, &e_);
+    if (rc != SQ_OK) throw SimError(rc, "sq_create_file_state failed (see the message above)");
 }
 
 }  // namespace sq

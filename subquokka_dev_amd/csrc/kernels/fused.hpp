@@ -146,10 +146,12 @@ __global__ void __launch_bounds__(256) k_fused5_small(cx<T> *__restrict__ s, con
 // basis vector e_c in the output buffer and applies the block's gates to it in
 // order.  No two threads touch the same element, so no shared memory is used.
 // ---------------------------------------------------------------------------
+// Every field is 32 bits wide: op records are read with scalar (SMEM) loads,
+// which drop the two low bits of their address components, so a dword field
+// behind byte fields must not be reached through a non-dword-aligned base.
 struct ComposeOp {
-    uint8_t kind;      // 0: dense 1q, 1: dense 2q, 2: swap
-    uint8_t t0, t1;    // local bits (t0 < t1)
-    uint8_t pad;
+    int32_t kind;      // 0: dense 1q, 1: dense 2q, 2: swap
+    int32_t t0, t1;    // local bits (t0 < t1)
     uint32_t coef;     // offset into the FP64 coefficient array
 };
 

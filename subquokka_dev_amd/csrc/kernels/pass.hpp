@@ -21,12 +21,15 @@
 
 namespace sqk {
 
-enum PassOpKind : uint8_t { PO_DENSE_REG = 0, PO_DENSE_LANE = 1, PO_DENSE_VEC = 2, PO_DIAG1 = 3, PO_DIAG2 = 4 };
+enum PassOpKind : int32_t { PO_DENSE_REG = 0, PO_DENSE_LANE = 1, PO_DENSE_VEC = 2, PO_DIAG1 = 3, PO_DIAG2 = 4 };
 
+// 32-bit fields only: the op loop reads records with scalar (SMEM) loads,
+// which ignore the two low bits of their address components; byte fields
+// beside the dword field let the compiler address it from a base 2 bytes off
+// alignment, and the load then returns the byte fields instead.
 struct PassOp {
-    uint8_t kind;
-    uint8_t a, b;      // DENSE_REG: register slot; DENSE_LANE: lane bit; DIAG: global bits (a < b)
-    uint8_t pad;
+    int32_t kind;
+    int32_t a, b;      // DENSE_REG: register slot; DENSE_LANE: lane bit; DIAG: global bits (a < b)
     uint32_t coef;     // offset of the op's coefficients (4 for dense, 2 or 4 for diagonal)
 };
 

@@ -14,12 +14,15 @@
 
 namespace sqk {
 
-enum TileKind : uint8_t { TK_DENSE1 = 0, TK_DENSE2 = 1, TK_DIAG1 = 2, TK_DIAG2 = 3, TK_SWAP = 4 };
+enum TileKind : int32_t { TK_DENSE1 = 0, TK_DENSE2 = 1, TK_DIAG1 = 2, TK_DIAG2 = 3, TK_SWAP = 4 };
 
+// 32-bit fields only: the op loop reads records with scalar (SMEM) loads,
+// which ignore the two low bits of their address components; byte fields
+// beside the dword field let the compiler address it from a base 2 bytes off
+// alignment, and the load then returns the byte fields instead.
 struct TileOp {
-    uint8_t kind;
-    uint8_t t0, t1;      // local target bits, t0 < t1
-    uint8_t pad;
+    int32_t kind;
+    int32_t t0, t1;      // local target bits, t0 < t1
     uint32_t coef;       // offset of the op's coefficients in the coefficient array
 };
 

@@ -74,7 +74,7 @@ int launch_tile(sq_engine *e, int K, const TileArgs<T> &a, unsigned grid) {
 // Appends one gate as a tile op (local bit positions) and its coefficients.
 template <typename T>
 void encode_op(const LoweredGate &g, const std::vector<int> &q, std::vector<TileOp> &ops, std::vector<cx<T>> &coef) {
-    auto local = [&](int t) { return (uint8_t)(std::lower_bound(q.begin(), q.end(), t) - q.begin()); };
+    auto local = [&](int t) { return (int32_t)(std::lower_bound(q.begin(), q.end(), t) - q.begin()); };
     auto push_coef = [&](int cnt) {
         for (int i = 0; i < cnt; i++) coef.push_back({(T)g.m[i].real(), (T)g.m[i].imag()});
     };
@@ -84,7 +84,7 @@ void encode_op(const LoweredGate &g, const std::vector<int> &q, std::vector<Tile
         for (int p = 0; p < g.n_pairs; p++) {
             TileOp w{};
             int a = local(g.t[2 * p]), b = local(g.t[2 * p + 1]);
-            w.kind = TK_SWAP; w.t0 = (uint8_t)std::min(a, b); w.t1 = (uint8_t)std::max(a, b);
+            w.kind = TK_SWAP; w.t0 = std::min(a, b); w.t1 = std::max(a, b);
             w.coef = (uint32_t)coef.size();
             ops.push_back(w);
         }

@@ -266,18 +266,26 @@ int sq_engine::ensure_partials() {
 // C ABI
 // ---------------------------------------------------------------------------
 
-extern "C" {
+namespace {
 
-int sq_create(const sq_config *cfg, sq_engine **out) {
+// sq_create / sq_create_file_state: files == nullptr for memory tiers.
+int create_engine(const sq_config *cfg, const char *const *files, int n_files, bool direct, bool keep,
+                  sq_engine **out) {
     if (!cfg || !out) return SQ_ERR_ARG;
     *out = nullptr;
     if (cfg->total_qbit < 1 || cfg->total_qbit > 36) return SQ_ERR_ARG;
     if (cfg->dtype != SQ_C128 && cfg->dtype != SQ_C64) return SQ_ERR_ARG;
     if (cfg->chunk_qbit < 0 || cfg->chunk_qbit > cfg->total_qbit) return SQ_ERR_ARG;
     if (cfg->sweep_mode < 0 || cfg->sweep_mode > 1 || cfg->pass_qbits < 0 || cfg->pass_qbits > 4) return SQ_ERR_ARG;
+    if (cfg->host_tier != SQ_HOST_AUTO && cfg->host_tier != SQ_HOST_PAGEABLE) return SQ_ERR_ARG;
     // a resident chunk at least as large as the state means the state fits in HBM
     const int resident = cfg->resident_qbit >= cfg->total_qbit ? 0 : cfg->resident_qbit;
     if (resident < 0) return SQ_ERR_ARG;
+    if (files) {
+        if (resident == 0 || n_files < 1 || (n_files & (n_files - 1)) || (1LL << cfg->total_qbit) < n_files)
+            return SQ_ERR_ARG;
+        for (int i = 0; i < n_files; i++) if (!files[i]) return SQ_ERR_ARG;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) return SQ_ERR_HIP;
     auto *e = new sq_engine();
@@ -293,8 +301,16 @@ int sq_create(const sq_config *cfg, sq_engine **out) {
         // out-of-core runner: resident chunk of 2^resident amplitudes
         if (resident < 8 || resident > 34) { hipStreamDestroy(e->stream); delete e; return SQ_ERR_ARG; }
         e->res_qbit = resident;
-        int rc = e->stream_create();
-        if (rc) { e->stream_destroy(); hipStreamDestroy(e->stream); delete e; return rc; }
+        e->host_tier = cfg->host_tier;
+        int rc = files ? e->stream_open_files(files, n_files, direct, keep) : SQ_OK;
+        if (!rc) rc = e->stream_create();
+        if (rc) {
+            std::fprintf(stderr, "[sq_create]: %s\n", e->err.c_str());
+            e->stream_destroy();
+            hipStreamDestroy(e->stream);
+            delete e;
+            return rc;
+        }
     } else if (hipMalloc(&e->d_state, e->state_bytes()) != hipSuccess) {
         hipGetLastError();
         hipStreamDestroy(e->stream);
@@ -308,7 +324,20 @@ int sq_create(const sq_config *cfg, sq_engine **out) {
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) == hipSuccess && lds > 0)
         e->lds_per_block = (size_t)lds;
     *out = e;
+    if (files && keep) return SQ_OK;
     return sq_init_zero(e);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sq_create(const sq_config *cfg, sq_engine **out) { return create_engine(cfg, nullptr, 0, false, false, out); }
+
+int sq_create_file_state(const sq_config *cfg, const char *const *paths, int32_t n_files, int32_t direct_io,
+                         int32_t keep_contents, sq_engine **out) {
+    if (!paths) return SQ_ERR_ARG;
+    return create_engine(cfg, paths, n_files, direct_io != 0, keep_contents != 0, out);
 }
 
 int sq_destroy(sq_engine *e) {
@@ -475,8 +504,9 @@ int sq_stats_reset(sq_engine *e) {
     return SQ_OK;
 }
 
-int sq_stream_info(sq_engine *e, uint64_t *passes, double *host_link_bytes) {
+int sq_stream_info(sq_engine *e, uint64_t *passes, double *host_link_bytes, int32_t *host_tier) {
     if (!e) return SQ_ERR_ARG;
+    if (host_tier) *host_tier = e->out_of_core() ? e->host_tier : SQ_HOST_AUTO;
     if (passes) *passes = e->n_stream_passes;
     // every pass moves the whole state host -> HBM and back
     if (host_link_bytes) *host_link_bytes = 2.0 * (double)e->n_stream_passes * (double)e->n_amps() * (double)e->amp_bytes();

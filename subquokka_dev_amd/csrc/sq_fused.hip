@@ -30,7 +30,7 @@ struct FusedProgram {
 
 // Appends the gate as compose ops on the local bits of q (ascending).
 void encode_compose(const LoweredGate &g, const std::vector<int> &q, FusedProgram &p) {
-    auto local = [&](int t) { return (uint8_t)(std::lower_bound(q.begin(), q.end(), t) - q.begin()); };
+    auto local = [&](int t) { return (int32_t)(std::lower_bound(q.begin(), q.end(), t) - q.begin()); };
     auto push = [&](const std::complex<double> *m, int cnt) {
         for (int i = 0; i < cnt; i++) p.coef.push_back({m[i].real(), m[i].imag()});
     };
@@ -38,7 +38,7 @@ void encode_compose(const LoweredGate &g, const std::vector<int> &q, FusedProgra
         for (int pr = 0; pr < g.n_pairs; pr++) {
             ComposeOp o{};
             int a = local(g.t[2 * pr]), b = local(g.t[2 * pr + 1]);
-            o.kind = 2; o.t0 = (uint8_t)std::min(a, b); o.t1 = (uint8_t)std::max(a, b);
+            o.kind = 2; o.t0 = std::min(a, b); o.t1 = std::max(a, b);
             o.coef = (uint32_t)p.coef.size();
             p.ops.push_back(o);
         }

@@ -50,15 +50,15 @@ void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vec
             o.kind = PO_DENSE_VEC;
         } else if (t < S) {
             o.kind = PO_DENSE_LANE;
-            o.a = (uint8_t)(t - VB);
+            o.a = t - VB;
         } else {
             o.kind = PO_DENSE_REG;
-            o.a = (uint8_t)(std::find(regs.begin(), regs.end(), t) - regs.begin());
+            o.a = (int32_t)(std::find(regs.begin(), regs.end(), t) - regs.begin());
         }
     } else {
         o.kind = g.k == 1 ? PO_DIAG1 : PO_DIAG2;
-        o.a = (uint8_t)g.t[0];
-        o.b = (uint8_t)(g.k == 2 ? g.t[1] : 0);
+        o.a = g.t[0];
+        o.b = g.k == 2 ? g.t[1] : 0;
     }
     ops.push_back(o);
 }

@@ -42,7 +42,7 @@ class SqFused(ctypes.Structure):
 class SqConfig(ctypes.Structure):
     _fields_ = [("total_qbit", ctypes.c_int32), ("chunk_qbit", ctypes.c_int32), ("dtype", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("resident_qbit", ctypes.c_int32), ("sweep_mode", ctypes.c_int32),
-                ("pass_qbits", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
+                ("pass_qbits", ctypes.c_int32), ("host_tier", ctypes.c_int32)]
 
 
 class SqKstat(ctypes.Structure):
@@ -85,7 +85,10 @@ _SIGS = {
     "sq_stats_enable": (ctypes.c_int, [_P, ctypes.c_int32]),
     "sq_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(SqKstat), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "sq_stats_reset": (ctypes.c_int, [_P]),
-    "sq_stream_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]),
+    "sq_stream_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_int32)]),
+    "sq_create_file_state": (ctypes.c_int, [ctypes.POINTER(SqConfig), ctypes.POINTER(ctypes.c_char_p), ctypes.c_int32,
+                                            ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_P)]),
     "sq_parse_gate": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(SqGate)]),
     "sq_parse_circuit": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(SqGate),
                                         ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
@@ -194,13 +197,22 @@ def plan(text, n, mode, param):
 class Engine:
     """One state on one GPU (owns device memory until close())."""
 
-    def __init__(self, n, dtype="c128", chunk=0, device=0, resident_qbit=0, sweep_mode=0, pass_qbits=0):
+    def __init__(self, n, dtype="c128", chunk=0, device=0, resident_qbit=0, sweep_mode=0, pass_qbits=0,
+                 host_tier="auto", state_files=None, direct_io=False, keep_files=False):
         """sweep_mode 0: plain sweeps run as register passes (pass_qbits register
-        qubits, 0 = default); 1: one launch per gate."""
+        qubits, 0 = default); 1: one launch per gate.  resident_qbit > 0: the
+        out-of-core runner, with the state in host memory (host_tier "auto" or
+        "pageable") or, with state_files (a list of 2^f paths), in files."""
         cfg = SqConfig(total_qbit=n, chunk_qbit=chunk, dtype=SQ_C128 if dtype == "c128" else SQ_C64,
-                       device=device, resident_qbit=resident_qbit, sweep_mode=sweep_mode, pass_qbits=pass_qbits)
+                       device=device, resident_qbit=resident_qbit, sweep_mode=sweep_mode, pass_qbits=pass_qbits,
+                       host_tier={"auto": 0, "pageable": 1}[host_tier])
         h = ctypes.c_void_p()
-        rc = lib().sq_create(ctypes.byref(cfg), ctypes.byref(h))
+        if state_files:
+            paths = (ctypes.c_char_p * len(state_files))(*[os.fsencode(p) for p in state_files])
+            rc = lib().sq_create_file_state(ctypes.byref(cfg), paths, len(state_files), int(direct_io),
+                                            int(keep_files), ctypes.byref(h))
+        else:
+            rc = lib().sq_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != SQ_OK:
             raise SqError(rc, "sq_create failed")
         self.h = h
@@ -306,10 +318,11 @@ class Engine:
         self._c(lib().sq_stats_reset(self.h))
 
     def stream_info(self):
-        """(passes, host-link bytes) of the out-of-core runner since stats_reset."""
-        p, b = ctypes.c_uint64(), ctypes.c_double()
-        self._c(lib().sq_stream_info(self.h, ctypes.byref(p), ctypes.byref(b)))
-        return p.value, b.value
+        """(passes, host-link bytes, host tier name) of the out-of-core runner
+        since stats_reset."""
+        p, b, t = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_int32()
+        self._c(lib().sq_stream_info(self.h, ctypes.byref(p), ctypes.byref(b), ctypes.byref(t)))
+        return p.value, b.value, {0: "hbm", 1: "pageable", 2: "pinned", 3: "files"}.get(t.value, "?")
 
     def stats(self):
         arr = (SqKstat * 64)()

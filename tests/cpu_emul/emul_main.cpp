@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <string>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -39,7 +40,20 @@ int main(int argc, char **argv) {
     if (ooc) cfg.resident_qbit = chunk;
     sq_engine *e = nullptr;
     const char *seed_env = std::getenv("EMUL_INIT_SEED");
-    if ((rc = sq_create(&cfg, &e))) { std::fprintf(stderr, "create rc=%d\n", rc); return 4; }
+    // out-of-core host tier: EMUL_HOST_TIER = pageable | files (files in EMUL_FILES_DIR)
+    const char *tier = std::getenv("EMUL_HOST_TIER");
+    std::vector<std::string> paths;
+    if (ooc && tier && std::string(tier) == "pageable") cfg.host_tier = SQ_HOST_PAGEABLE;
+    if (ooc && tier && std::string(tier) == "files") {
+        const char *dir = std::getenv("EMUL_FILES_DIR");
+        for (int i = 0; i < 4; i++) paths.push_back(std::string(dir ? dir : ".") + "/state" + std::to_string(i) + ".bin");
+        std::vector<const char *> cp;
+        for (auto &p : paths) cp.push_back(p.c_str());
+        rc = sq_create_file_state(&cfg, cp.data(), 4, 0, 0, &e);
+    } else {
+        rc = sq_create(&cfg, &e);
+    }
+    if (rc) { std::fprintf(stderr, "create rc=%d\n", rc); return 4; }
     if (mode == "adjoint") {   // chunk argument = layers; theta from a fixed formula
         const int layers = chunk;
         std::vector<double> theta((size_t)2 * n * layers), grad(theta.size());

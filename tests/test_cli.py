@@ -47,6 +47,7 @@ def test_missing_ini(tmp_path):
     (dict(total_qbit=6, chunk_qbit=2, mpi_qbit=1, runner_type="MPI_MEM"), "mpi_qbit"),
     (dict(total_qbit=6, chunk_qbit=2, runner_type="MEM", precision="c32"), "precision"),
     (dict(total_qbit=6, chunk_qbit=2, runner_type="MEM", fused_k=7), "fused_k"),
+    (dict(total_qbit=8, chunk_qbit=2, file_qbit=2, runner_type="DirectIO", state_paths="a,b"), "state_paths"),
 ])
 def test_config_errors(tmp_path, kv, msg):
     write_ini(tmp_path / "a.ini", **kv)
@@ -77,6 +78,26 @@ def test_cli_plain_matches_oracle(tmp_path, oracle, runner):
     s0 = sq.gen_state(n, 7)
     np.testing.assert_allclose(dumped(tmp_path / "out.bin", n), oracle.run(s0, n, text, chunk=5),
                                rtol=0, atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("runner", ["DirectIO", "IO"])
+def test_cli_state_files_hold_the_result(tmp_path, oracle, runner):
+    # runner_type DirectIO / IO with state_paths: the state lives in 2^file_qbit
+    # files (setupFiles, simulator.cpp:102-129) and holds the result afterwards
+    n = 16
+    text = random_circuit(n, 120, seed=5)
+    open(tmp_path / "c.txt", "w").write(text)
+    paths = ",".join(f"./state/path{i}" for i in range(4))
+    write_ini(tmp_path / "a.ini", total_qbit=n, file_qbit=2, chunk_qbit=5, runner_type=runner,
+              is_subcircuit=0, init_seed=9, resident_qbit=14, state_paths=paths)
+    r = run(["-i", "a.ini", "-c", "c.txt"], tmp_path)
+    if r.returncode != 0 and runner == "DirectIO" and "create failed" in r.stderr:
+        pytest.skip("O_DIRECT is not supported on this filesystem: " + r.stderr[-200:])
+    assert r.returncode == 0, r.stderr
+    assert ("DIO Mode" if runner == "DirectIO" else "IO Mode") in r.stdout
+    got = np.concatenate([np.fromfile(tmp_path / "state" / f"path{i}", dtype=np.complex128) for i in range(4)])
+    np.testing.assert_allclose(got, oracle.run(sq.gen_state(n, 9), n, text, chunk=5), rtol=0, atol=1e-12)
 
 
 @pytest.mark.gpu

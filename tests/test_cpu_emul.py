@@ -29,13 +29,16 @@ def emul():
     return EMUL
 
 
-def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None):
+def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None, tier=None):
     f = tmp_path / "c.txt"
     f.write_text(text)
-    out = tmp_path / "state.bin"
+    out = tmp_path / f"state_{tier or 'default'}.bin"
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
     if seed is not None:
         env["EMUL_INIT_SEED"] = str(seed)
+    if tier is not None:
+        env["EMUL_HOST_TIER"] = tier
+        env["EMUL_FILES_DIR"] = str(tmp_path)
     r = subprocess.run([emul, mode, str(n), str(chunk), dtype, str(f), str(out)], capture_output=True, text=True,
                        timeout=900, env=env)
     assert r.returncode == 0, (r.stdout + r.stderr)[-3000:]
@@ -112,3 +115,18 @@ def test_adjoint_launches(emul, tmp_path, dtype, n, layers):
 def test_out_of_core_launches(emul, tmp_path, dtype, n, w):
     text = sq.gen_circuit("random", n, 3, seed=n) + random_circuit(n, 60, seed=w)
     run_case(emul, tmp_path, "ooc", n, w, dtype, text)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n,w", [(12, 9), (14, 10), (15, 12)])
+def test_out_of_core_staged_tiers_match_pinned(emul, tmp_path, dtype, n, w):
+    # the emulated kernels are deterministic, so every host tier (pinned
+    # state, pageable state + staging, state files + staging) must leave the
+    # same bits: this checks the gather/scatter and file addressing on the CPU
+    text = sq.gen_circuit("random", n, 3, seed=n) + random_circuit(n, 60, seed=w)
+    base = run_case(emul, tmp_path, "ooc", n, w, dtype, text, seed=7)
+    for tier in ("pageable", "files"):
+        got = run_case(emul, tmp_path, "ooc", n, w, dtype, text, seed=7, tier=tier)
+        assert np.array_equal(got, base), tier
+    files = sorted(tmp_path.glob("state[0-9].bin"))
+    assert len(files) == 4 and all(f.stat().st_size == (16 if dtype == "c128" else 8) << (n - 2) for f in files)

@@ -55,3 +55,62 @@ def test_out_of_core_rejects_hbm_only_entry_points():
             eng.hea_energy(1, np.zeros(2 * n))
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_pageable_tier_matches_oracle(oracle, dtype):
+    # state in ordinary host memory, groups packed through pinned staging
+    n, w = 16, 12
+    text = sq.gen_circuit("random", n, 3, seed=2) + random_circuit(n, 100, seed=21)
+    s0 = random_state(n, 5)
+    eng = sq.Engine(n, dtype=dtype, resident_qbit=w, host_tier="pageable")
+    try:
+        eng.set_state(s0.astype(np.complex128 if dtype == "c128" else np.complex64))
+        eng.apply_text(text)
+        got = eng.get_state()
+        passes, _, tier = eng.stream_info()
+    finally:
+        eng.close()
+    assert tier == "pageable" and passes >= 1
+    tol = 1e-12 if dtype == "c128" else 2e-6
+    assert np.max(np.abs(got - oracle.run(s0, n, text))) <= tol
+
+
+def read_state_files(paths, dtype=np.complex128):
+    return np.concatenate([np.fromfile(p, dtype=dtype) for p in paths])
+
+
+@pytest.mark.parametrize("direct", [False, True])
+def test_file_tier_matches_oracle_and_resumes(oracle, tmp_path, direct):
+    # state in 4 files of the reference's layout; a second engine reopens the
+    # files and continues the circuit from them
+    n, w = 16, 14
+    paths = [str(tmp_path / f"path{i}") for i in range(4)]
+    t1 = sq.gen_circuit("random", n, 2, seed=3) + random_circuit(n, 60, seed=31)
+    t2 = random_circuit(n, 60, seed=32)
+    s0 = random_state(n, 6)
+    try:
+        eng = sq.Engine(n, resident_qbit=w, state_files=paths, direct_io=direct)
+    except sq.SqError as e:
+        if direct and e.code == -9:
+            pytest.skip("O_DIRECT is not supported on this filesystem")
+        raise
+    try:
+        eng.set_state(s0)
+        eng.apply_text(t1)
+        assert eng.stream_info()[2] == "files"
+    finally:
+        eng.close()
+    mid = oracle.run(s0, n, t1)
+    assert np.max(np.abs(read_state_files(paths) - mid)) <= 1e-12
+    eng = sq.Engine(n, resident_qbit=w, state_files=paths, direct_io=direct, keep_files=True)
+    try:
+        eng.apply_text(t2)
+        got = eng.get_state()
+        zz = eng.expect_zz_ring()
+    finally:
+        eng.close()
+    want = oracle.run(s0, n, t1 + t2)
+    assert np.max(np.abs(got - want)) <= 1e-12
+    assert np.max(np.abs(read_state_files(paths) - want)) <= 1e-12
+    assert abs(zz - oracle.expect_zz_ring(want, n)) < 1e-10
