@@ -165,6 +165,14 @@ int sq_inner(sq_engine *a, sq_engine *b, double *re, double *im);
 /* max_i |a_i - b_i| over two engines of the same size. */
 int sq_max_abs_diff(sq_engine *a, sq_engine *b, double *out);
 
+/* ---- measurement (no reference counterpart: only the unused MEASURE gate
+ * type, circuit.hpp:11) ----
+ * shots samples of the computational-basis distribution |a_i|^2 / sum |a|^2
+ * of the HBM-resident state, driven by SplitMix64(seed); out[k] = index of
+ * shot k.  The rule is exact and reproducible (FP64 sums in a fixed order;
+ * see kernels/sample.hpp), so equal seeds give equal shots. */
+int sq_sample(sq_engine *e, uint64_t seed, uint64_t shots, uint64_t *out);
+
 /* ---- adjoint gradient (no reference counterpart) ----
  * Hardware-efficient ansatz: layers x (RY(theta) on every qubit, RZ(theta) on
  * every qubit, CNOT ring i -> (i+1) mod N); theta has 2*N*layers entries in

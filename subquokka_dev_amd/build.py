@@ -28,7 +28,8 @@ HIP_FLAGS = [
 ]
 INCLUDES = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
-HIP_SOURCES = ["sq_engine.hip", "sq_blocks.hip", "sq_fused.hip", "sq_adjoint.hip", "sq_stream.hip", "sq_pass.hip"]
+HIP_SOURCES = ["sq_engine.hip", "sq_blocks.hip", "sq_fused.hip", "sq_adjoint.hip", "sq_stream.hip", "sq_pass.hip",
+               "sq_sample.hip"]
 HOST_SOURCES = ["host/formats.cpp", "host/simulator.cpp", "host/planner.cpp"]
 CLI_SOURCES = ["host/quokka_main.cpp"]
 

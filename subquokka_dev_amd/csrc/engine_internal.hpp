@@ -44,6 +44,8 @@ struct sq_engine {
     void *d_gradpart = nullptr;     // adjoint: per-parameter block partials
     size_t gradpart_cap = 0;
     uint64_t n_tile_passes = 0;     // tile passes launched (diagnostics)
+    void *d_sample = nullptr;       // sampler: block sums, shots, results
+    size_t sample_cap = 0;
     void *d_passprog = nullptr;     // register-pass program of the plain sweep
     size_t passprog_cap = 0;
     int sweep_mode = 0;             // 0: register passes, 1: one launch per gate

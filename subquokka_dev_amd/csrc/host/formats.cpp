@@ -15,6 +15,7 @@
 
 #include "subquokka_hip.h"
 #include "gate_lowering.hpp"
+#include "rng.hpp"
 
 namespace {
 
@@ -50,17 +51,7 @@ bool read_double(const char *&p, double &v) {
 }
 bool is_blank(const std::string &s) { return s.find_first_not_of(" \t\r") == std::string::npos; }
 
-// SplitMix64 (Steele, Lea, Flood 2014).
-struct SplitMix64 {
-    uint64_t x;
-    uint64_t next() {
-        uint64_t z = (x += 0x9E3779B97F4A7C15ULL);
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-        return z ^ (z >> 31);
-    }
-    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }   // [0, 1)
-};
+using sq::SplitMix64;
 
 void put(std::string &s, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void put(std::string &s, const char *fmt, ...) {

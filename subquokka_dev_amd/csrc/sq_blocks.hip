@@ -239,10 +239,10 @@ extern "C" int sq_apply_blocks(sq_engine *e, const sq_gate *gates, const uint32_
 }
 
 void sq_engine::free_extras() {
-    for (void **p : {&d_blockprog, &d_fused, &d_aux, &d_ckpt, &d_gradpart, &d_passprog}) {
+    for (void **p : {&d_blockprog, &d_fused, &d_aux, &d_ckpt, &d_gradpart, &d_passprog, &d_sample}) {
         if (*p) hipFree(*p);
         *p = nullptr;
     }
-    blockprog_cap = fused_cap = aux_cap = ckpt_cap = gradpart_cap = passprog_cap = 0;
+    blockprog_cap = fused_cap = aux_cap = ckpt_cap = gradpart_cap = passprog_cap = sample_cap = 0;
 }
 

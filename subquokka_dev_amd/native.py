@@ -82,6 +82,7 @@ _SIGS = {
                                      ctypes.POINTER(ctypes.c_double)]),
     "sq_adjoint_grad_hea": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_int32,
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "sq_sample": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _P]),
     "sq_stats_enable": (ctypes.c_int, [_P, ctypes.c_int32]),
     "sq_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(SqKstat), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "sq_stats_reset": (ctypes.c_int, [_P]),
@@ -316,6 +317,12 @@ class Engine:
 
     def stats_reset(self):
         self._c(lib().sq_stats_reset(self.h))
+
+    def sample(self, shots, seed):
+        """Seeded computational-basis shots (uint64 indices)."""
+        out = np.empty(max(1, shots), dtype=np.uint64)
+        self._c(lib().sq_sample(self.h, seed, shots, out.ctypes.data))
+        return out[:shots]
 
     def stream_info(self):
         """(passes, host-link bytes, host tier name) of the out-of-core runner

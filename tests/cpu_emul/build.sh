@@ -8,6 +8,6 @@ ROOT="$(cd "$HERE/../.." && pwd)"
 CSRC="$ROOT/subquokka_dev_amd/csrc"
 /opt/rocm/llvm/bin/clang++ -std=c++17 -O1 -g -fsanitize=address -fno-omit-frame-pointer \
   -DSQ_TILE_PASSES=1 -I"$HERE" -I"$ROOT/include" -I"$CSRC" \
-  -x c++ "$CSRC/sq_engine.hip" -x c++ "$CSRC/sq_blocks.hip" -x c++ "$CSRC/sq_fused.hip" -x c++ "$CSRC/sq_adjoint.hip" -x c++ "$CSRC/sq_stream.hip" -x c++ "$CSRC/sq_pass.hip" \
+  -x c++ "$CSRC/sq_engine.hip" -x c++ "$CSRC/sq_blocks.hip" -x c++ "$CSRC/sq_fused.hip" -x c++ "$CSRC/sq_adjoint.hip" -x c++ "$CSRC/sq_stream.hip" -x c++ "$CSRC/sq_pass.hip" -x c++ "$CSRC/sq_sample.hip" \
   "$CSRC/host/formats.cpp" "$CSRC/host/planner.cpp" "$HERE/emul_main.cpp" \
   -o "$HERE/emul"

@@ -84,6 +84,11 @@ int main(int argc, char **argv) {
     }
     double nrm = 0;
     if ((rc = sq_norm2(e, &nrm))) return 7;
+    if (!ooc && nrm > 0) {   // measurement shots (addresses only: wave-shared values are not emulated)
+        std::vector<uint64_t> shots(777);
+        if ((rc = sq_sample(e, 5, shots.size(), shots.data()))) { std::fprintf(stderr, "sample rc=%d %s\n", rc, sq_last_error(e)); return 12; }
+        for (uint64_t v : shots) if (v >> n) { std::fprintf(stderr, "shot out of range\n"); return 13; }
+    }
     sq_destroy(e);
     std::printf("ok gates=%llu blocks=%llu\n", (unsigned long long)ng, (unsigned long long)nb);
     return 0;

@@ -248,3 +248,77 @@ def qft_closed_form(state, n):
 def infidelity(a, b):
     ov = np.vdot(a, b)
     return 1.0 - abs(ov) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+# ---------------------------------------------------------------------------
+# measurement sampling: CPU restatement of the rule in kernels/sample.hpp
+# ---------------------------------------------------------------------------
+
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(seed, count):
+    """First `count` outputs of SplitMix64(seed) (Steele, Lea, Flood 2014)."""
+    x = seed & MASK64
+    out = []
+    for _ in range(count):
+        x = (x + 0x9E3779B97F4A7C15) & MASK64
+        z = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def sample_oracle(state, seed, shots):
+    """Shots of the engine's sampler, restated step by step in FP64 with the
+    same summation order (sequential sums of 16-amplitude segments, of 64
+    segments per block, and over blocks)."""
+    st = np.asarray(state)
+    n = int(np.log2(st.size))
+    re = st.real.astype(np.float64)
+    im = st.imag.astype(np.float64)
+    p = re * re + im * im
+    seg = p.reshape(-1, 16)
+    s = np.zeros(seg.shape[0])
+    for j in range(16):
+        s = s + seg[:, j]
+    spb = 64 if n >= 10 else 1 << (n - 4)
+    sb = s.reshape(-1, spb)
+    B = np.zeros(sb.shape[0])
+    for l in range(spb):
+        B = B + sb[:, l]
+    inc = np.cumsum(B)
+    C = np.concatenate([[0.0], inc[:-1]])
+    total = inc[-1]
+    nz = np.nonzero(B > 0)[0]
+    last_nz = int(nz[-1])
+    out = []
+    for x in splitmix64(seed, shots):
+        u = (float(x >> 11) * 2.0 ** -53) * total
+        b = int(np.searchsorted(inc, u, side="right"))
+        if b >= B.size:
+            b = last_nz
+        r = u - C[b]
+        L, m, Lm, last, Llast = 0.0, -1, 0.0, 0, 0.0
+        for l in range(spb):
+            sl = sb[b, l]
+            if sl > 0.0:
+                last, Llast = l, L
+            if m < 0 and L + sl > r:
+                m, Lm = l, L
+            L = L + sl
+        if m < 0:
+            m, Lm = last, Llast
+        t = r - Lm
+        a, j, jlast = 0.0, -1, 0
+        for i in range(16):
+            pi = p[(b * spb + m) * 16 + i]
+            if pi > 0.0:
+                jlast = i
+            a = a + pi
+            if j < 0 and a > t:
+                j = i
+        if j < 0:
+            j = jlast
+        out.append((b * spb + m) * 16 + j)
+    return np.array(out, dtype=np.uint64)
