@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -59,6 +60,8 @@ struct sq_engine {
     void *h_stage[2] = {nullptr, nullptr};   // pinned staging (PAGEABLE / FILES tiers)
     std::vector<int> file_fds;      // FILES tier: file i = amplitudes [i 2^n/nf, (i+1) 2^n/nf)
     bool file_direct = false;
+    std::atomic<int> io_errno{0};   // errno of the last failed state-file transfer (any host thread)
+    int io_fail(const char *what);
     void *d_group[2] = {nullptr, nullptr};
     hipEvent_t ev_loaded[2] = {nullptr, nullptr}, ev_computed[2] = {nullptr, nullptr},
                ev_stored[2] = {nullptr, nullptr}, ev_pass = nullptr;

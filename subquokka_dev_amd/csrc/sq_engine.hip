@@ -323,9 +323,16 @@ int create_engine(const sq_config *cfg, const char *const *files, int n_files, b
     int lds = 65536;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) == hipSuccess && lds > 0)
         e->lds_per_block = (size_t)lds;
+    if (!(files && keep)) {
+        const int rc = sq_init_zero(e);
+        if (rc) {
+            std::fprintf(stderr, "[sq_create]: %s\n", e->err.c_str());
+            sq_destroy(e);
+            return rc;
+        }
+    }
     *out = e;
-    if (files && keep) return SQ_OK;
-    return sq_init_zero(e);
+    return SQ_OK;
 }
 
 }  // namespace

@@ -71,25 +71,36 @@ LoweredGate remap(const LoweredGate &g, const std::vector<int> &P) {
     return r;
 }
 
-// Whole-range pread/pwrite (short transfers continue; errors return false).
-bool pread_all(int fd, char *dst, size_t bytes, off_t off) {
+// Whole-range pread/pwrite (short transfers continue).  On failure *err holds
+// errno (EIO for an unexpected end of file).
+bool pread_all(int fd, char *dst, size_t bytes, off_t off, int *err) {
     while (bytes) {
         const ssize_t r = pread(fd, dst, bytes, off);
-        if (r <= 0) { if (r < 0 && errno == EINTR) continue; return false; }
+        if (r < 0 && errno == EINTR) continue;
+        if (r <= 0) { *err = r < 0 ? errno : EIO; return false; }
         dst += r; bytes -= (size_t)r; off += r;
     }
     return true;
 }
-bool pwrite_all(int fd, const char *src, size_t bytes, off_t off) {
+bool pwrite_all(int fd, const char *src, size_t bytes, off_t off, int *err) {
     while (bytes) {
         const ssize_t r = pwrite(fd, src, bytes, off);
-        if (r <= 0) { if (r < 0 && errno == EINTR) continue; return false; }
+        if (r < 0 && errno == EINTR) continue;
+        if (r <= 0) { *err = r < 0 ? errno : EIO; return false; }
         src += r; bytes -= (size_t)r; off += r;
     }
     return true;
 }
 
 }  // namespace
+
+// Failure message of a state-file transfer, with the system's reason.
+int sq_engine::io_fail(const char *what) {
+    const int err = io_errno.load();
+    std::string m = std::string(what) + ": " + std::strerror(err ? err : EIO);
+    if (file_direct) m += " (O_DIRECT)";
+    return fail(SQ_ERR_IO, m);
+}
 
 int sq_engine::stream_high() const { return std::min(kMaxHigh, std::min(res_qbit - 1, n - res_qbit + 2)); }
 
@@ -197,7 +208,8 @@ bool sq_engine::host_read(uint64_t off, size_t bytes, void *dst) {
     while (bytes) {
         const uint64_t f = off / per_file, fo = off % per_file;
         const size_t take = (size_t)std::min<uint64_t>(bytes, per_file - fo);
-        if (!pread_all(file_fds[f], d, take, (off_t)fo)) return false;
+        int err = 0;
+        if (!pread_all(file_fds[f], d, take, (off_t)fo, &err)) { io_errno = err; return false; }
         d += take; off += take; bytes -= take;
     }
     return true;
@@ -213,7 +225,8 @@ bool sq_engine::host_write(uint64_t off, size_t bytes, const void *src) {
     while (bytes) {
         const uint64_t f = off / per_file, fo = off % per_file;
         const size_t take = (size_t)std::min<uint64_t>(bytes, per_file - fo);
-        if (!pwrite_all(file_fds[f], s, take, (off_t)fo)) return false;
+        int err = 0;
+        if (!pwrite_all(file_fds[f], s, take, (off_t)fo, &err)) { io_errno = err; return false; }
         s += take; off += take; bytes -= take;
     }
     return true;
@@ -260,9 +273,9 @@ int sq_engine::host_chunks(bool write_back, F fn) {
     const size_t B = amp_bytes();
     for (uint64_t a = 0; a < na; a += chunk) {
         std::vector<uint64_t> one{0};
-        if (!stage_runs(true, a * B, one, chunk * B, buf)) return fail(SQ_ERR_IO, "state file read");
+        if (!stage_runs(true, a * B, one, chunk * B, buf)) return io_fail("state file read");
         fn(a, chunk, buf);
-        if (write_back && !stage_runs(false, a * B, one, chunk * B, buf)) return fail(SQ_ERR_IO, "state file write");
+        if (write_back && !stage_runs(false, a * B, one, chunk * B, buf)) return io_fail("state file write");
     }
     return SQ_OK;
 }
@@ -279,7 +292,7 @@ int sq_engine::stream_init_zero() {
         parallel_for(chunk * B, [&](uint64_t a, uint64_t b) { std::memset(buf + a, 0, b - a); });
         std::vector<uint64_t> one{0};
         for (uint64_t a = 0; a < n_amps(); a += chunk)
-            if (!stage_runs(false, a * B, one, chunk * B, buf)) return fail(SQ_ERR_IO, "state file write");
+            if (!stage_runs(false, a * B, one, chunk * B, buf)) return io_fail("state file write");
     }
     // amplitude 0 = 1 (MEM_Runner.cpp:79-81)
     alignas(16) unsigned char one[16] = {};
@@ -291,7 +304,7 @@ int sq_engine::stream_init_zero() {
         char *buf = static_cast<char *>(h_stage[0]);
         std::memset(buf, 0, kIoAlign);
         std::memcpy(buf, one, B);
-        if (!host_write(0, std::min<size_t>(kIoAlign, state_bytes()), buf)) return fail(SQ_ERR_IO, "state file write");
+        if (!host_write(0, std::min<size_t>(kIoAlign, state_bytes()), buf)) return io_fail("state file write");
     }
     return SQ_OK;
 }
@@ -420,9 +433,9 @@ int sq_engine::stream_apply(const std::vector<sq_gate> &gates, const std::vector
             char *dbuf = static_cast<char *>(d_group[b]);
             if (staged) {
                 // the staging buffer's previous group goes back first, then this group is packed into it
-                if (!scatter(b)) { rc = fail(SQ_ERR_IO, "out-of-core scatter"); break; }
+                if (!scatter(b)) { rc = io_fail("out-of-core scatter"); break; }
                 if (!stage_runs(true, base * B, run_off_bytes, run * B, static_cast<char *>(h_stage[b]))) {
-                    rc = fail(SQ_ERR_IO, "out-of-core gather");
+                    rc = io_fail("out-of-core gather");
                     break;
                 }
             }
@@ -457,7 +470,7 @@ int sq_engine::stream_apply(const std::vector<sq_gate> &gates, const std::vector
             }
             hipEventRecord(ev_stored[b], s_d2h);
         }
-        if (!rc && staged && (!scatter(0) || !scatter(1))) rc = fail(SQ_ERR_IO, "out-of-core scatter");
+        if (!rc && staged && (!scatter(0) || !scatter(1))) rc = io_fail("out-of-core scatter");
         if (rc) break;
         n_stream_passes++;
     }
