@@ -13,24 +13,40 @@
 //   * diagonal one- and two-qubit gates on ANY qubits (Z P RZ CP RZZ): the
 //     phase depends only on the amplitude's global index, which every lane
 //     knows.
-// Each gate uses the same operation sequence as the single-gate kernels
-// (k_dense: acc = 0, then cfma over the row; k_diag: cmul(amp, d)), so a pass
-// computes the same values as the gates applied one launch at a time.
+// Each gate is its own 2x2 (or diagonal) step in FP arithmetic of the state's
+// type -- nothing is composed -- so a pass agrees with the gates applied one
+// launch at a time up to the rounding of individual products.  Per-op work
+// is shaped so that nothing is selected per amplitude: lane-gate
+// coefficients are chosen once per lane, shuffle masks are constants, and a
+// diagonal factor is chosen per lane, per register at compile time, or once
+// per wave (identity entries then skip the op).
 #pragma once
 #include "common.hpp"
 
 namespace sqk {
 
-enum PassOpKind : int32_t { PO_DENSE_REG = 0, PO_DENSE_LANE = 1, PO_DENSE_VEC = 2, PO_DIAG1 = 3, PO_DIAG2 = 4 };
+enum PassOpKind : int32_t {
+    PO_DENSE_REG = 0,    // a = register slot of the target
+    PO_DENSE_LANE = 1,   // a = lane bit of the target (0..5)
+    PO_DENSE_VEC = 2,    // target inside the 16-byte vector (complex<float>, bit 0)
+    PO_DIAG_FREE = 3,    // diagonal, no bit in a register slot: a, b = global bits (b < 0: one-qubit);
+                         //   c = 1 when both bits are wave-uniform (tile bits), so identity
+                         //   entries skip the whole op
+    PO_DIAG_REG1 = 4,    // diagonal, one bit in register slot a, the other at global bit b (< 0: none);
+                         //   coefficients e[(register bit << 1) | other bit]
+    PO_DIAG_REG2 = 5,    // diagonal, both bits in register slots a < b
+};
 
 // 32-bit fields only: the op loop reads records with scalar (SMEM) loads,
 // which ignore the two low bits of their address components; byte fields
-// beside the dword field let the compiler address it from a base 2 bytes off
+// beside a dword field let the compiler address it from a base 2 bytes off
 // alignment, and the load then returns the byte fields instead.
 struct PassOp {
     int32_t kind;
-    int32_t a, b;      // DENSE_REG: register slot; DENSE_LANE: lane bit; DIAG: global bits (a < b)
-    uint32_t coef;     // offset of the op's coefficients (4 for dense, 2 or 4 for diagonal)
+    int32_t a, b, c;
+    uint32_t coef;       // offset of the op's coefficients (4 for dense and diagonal)
+    uint32_t active;     // diagonal: bitmask of entries that are not 1
+    int32_t pad[2];
 };
 
 constexpr int kPassMaxKH = 4;   // five register qubits spill on gfx950
@@ -64,24 +80,43 @@ __device__ inline void pass_dense_reg(Pack<T> (&v)[R], const cx<T> m0, const cx<
     }
 }
 
-// Value of global index bit `pos` for register r, vector element e of a lane
-// whose tile base (lane bits included) is `base`: the bit is a register slot,
-// a bit inside the 16-byte vector, or a bit of the base.
-template <int KH> struct BitSrc {
-    int slot;     // register slot holding the bit, or -1
-    int vec;      // 1: the bit is the vector element index
-    bool fixed;   // the bit's value when it comes from the base
-    __device__ BitSrc(int pos, const int *hpos, uint64_t base, int vb) {
-        slot = -1;
+// Dense 2x2 on lane bit MASK: a lane whose bit is 0 holds x0 and gets
+// m0 x0 + m1 x1; its partner holds x1 and gets m3 x1 + m2 x0.  The per-lane
+// coefficients are chosen once per op; the shuffle mask is a constant.
+template <typename T, int MASK, int R, int E>
+__device__ inline void pass_dense_lane(Pack<T> (&v)[R], bool hi, const cx<T> *c) {
+    const cx<T> ms = hi ? c[3] : c[0], mp = hi ? c[2] : c[1];
 #pragma unroll
-        for (int j = 0; j < KH; j++) if (hpos[j] == pos) slot = j;
-        vec = pos < vb;
-        fixed = (base >> pos) & 1;
+    for (int r = 0; r < R; r++) {
+        const Pack<T> p = shfl_xor_pack(v[r], MASK);
+#pragma unroll
+        for (int e = 0; e < E; e++) v[r].e[e] = cfma(cmul(ms, v[r].e[e]), mp, p.e[e]);
     }
-    __device__ bool get(int r, int e) const {
-        return slot >= 0 ? ((r >> slot) & 1) : (vec ? (e & 1) : fixed);
+}
+
+// Diagonal with a bit in register slot J: ph0 / ph1 are the lane's factors
+// for register bit 0 / 1 (per vector element).
+template <typename T, int J, int R, int E>
+__device__ inline void pass_diag_reg1(Pack<T> (&v)[R], const cx<T> (&ph0)[E], const cx<T> (&ph1)[E]) {
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], ((r >> J) & 1) ? ph1[e] : ph0[e]);
+}
+
+// Diagonal on register slots J0 < J1: the factor of register r is d[l(r)],
+// known per r at compile time; entries equal to 1 are skipped.
+template <typename T, int J0, int J1, int R, int E>
+__device__ inline void pass_diag_reg2(Pack<T> (&v)[R], const cx<T> *d, uint32_t active) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int l = ((r >> J0) & 1) | (((r >> J1) & 1) << 1);
+        if (!((active >> l) & 1)) continue;
+        const cx<T> f = d[l];
+#pragma unroll
+        for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], f);
     }
-};
+}
 
 template <typename T, int KH>
 __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const PassArgs<T> a) {
@@ -127,21 +162,15 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 }
                 break; }
             case PO_DENSE_LANE: {
-                // lane bit 0: out = m0*self + m1*partner; lane bit 1: out = m2*partner + m3*self
                 const bool hi = (lane >> o.a) & 1;
-                const cx<T> mA = hi ? c[2] : c[0], mB = hi ? c[3] : c[1];
-                const int mask = 1 << o.a;
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const Pack<T> p = shfl_xor_pack(v[r], mask);
-#pragma unroll
-                    for (int e = 0; e < E; e++) {
-                        const cx<T> first = hi ? p.e[e] : v[r].e[e], second = hi ? v[r].e[e] : p.e[e];
-                        cx<T> acc = {0, 0};
-                        acc = cfma(acc, mA, first);
-                        acc = cfma(acc, mB, second);
-                        v[r].e[e] = acc;
-                    }
+                switch (o.a) {
+                case 0: pass_dense_lane<T, 1, R, E>(v, hi, c); break;
+                case 1: pass_dense_lane<T, 2, R, E>(v, hi, c); break;
+                case 2: pass_dense_lane<T, 4, R, E>(v, hi, c); break;
+                case 3: pass_dense_lane<T, 8, R, E>(v, hi, c); break;
+                case 4: pass_dense_lane<T, 16, R, E>(v, hi, c); break;
+                case 5: pass_dense_lane<T, 32, R, E>(v, hi, c); break;
+                default: break;
                 }
                 break; }
             case PO_DENSE_VEC: {
@@ -160,25 +189,56 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                     }
                 }
                 break; }
-            case PO_DIAG1: {
-                const cx<T> d0 = c[0], d1 = c[1];
-                const BitSrc<KH> ba(o.a, a.hpos, base, VB);
+            case PO_DIAG_FREE: {
+                // the factor depends only on lane, tile and vector bits: one per lane (and element)
+                if (o.c) {   // both bits are tile bits: one entry for the whole wave
+                    const int l = (int)((base >> o.a) & 1) | (o.b >= 0 ? (int)((base >> o.b) & 1) << 1 : 0);
+                    if (!((o.active >> l) & 1)) break;
+                    const cx<T> f = c[l];
 #pragma unroll
-                for (int r = 0; r < R; r++) {
+                    for (int r = 0; r < R; r++)
 #pragma unroll
-                    for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], ba.get(r, e) ? d1 : d0);
+                        for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], f);
+                    break;
+                }
+                cx<T> f[E];
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const uint64_t idx = base + e;
+                    const int x = (int)((idx >> o.a) & 1), y = o.b >= 0 ? (int)((idx >> o.b) & 1) : 0;
+                    f[e] = y ? (x ? c[3] : c[2]) : (x ? c[1] : c[0]);
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++)
+#pragma unroll
+                    for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], f[e]);
+                break; }
+            case PO_DIAG_REG1: {
+                cx<T> ph0[E], ph1[E];
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const uint64_t idx = base + e;
+                    const int y = o.b >= 0 ? (int)((idx >> o.b) & 1) : 0;
+                    ph0[e] = y ? c[1] : c[0];
+                    ph1[e] = y ? c[3] : c[2];
+                }
+                switch (o.a) {
+                case 0: pass_diag_reg1<T, 0, R, E>(v, ph0, ph1); break;
+                case 1: if constexpr (KH > 1) pass_diag_reg1<T, 1, R, E>(v, ph0, ph1); break;
+                case 2: if constexpr (KH > 2) pass_diag_reg1<T, 2, R, E>(v, ph0, ph1); break;
+                case 3: if constexpr (KH > 3) pass_diag_reg1<T, 3, R, E>(v, ph0, ph1); break;
+                default: break;
                 }
                 break; }
-            case PO_DIAG2: {
-                const cx<T> d0 = c[0], d1 = c[1], d2 = c[2], d3 = c[3];
-                const BitSrc<KH> ba(o.a, a.hpos, base, VB), bb(o.b, a.hpos, base, VB);
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-#pragma unroll
-                    for (int e = 0; e < E; e++) {
-                        const bool x = ba.get(r, e), y = bb.get(r, e);
-                        v[r].e[e] = cmul(v[r].e[e], y ? (x ? d3 : d2) : (x ? d1 : d0));
-                    }
+            case PO_DIAG_REG2: {
+                switch (o.a * 4 + o.b) {
+                case 1: if constexpr (KH > 1) pass_diag_reg2<T, 0, 1, R, E>(v, c, o.active); break;
+                case 2: if constexpr (KH > 2) pass_diag_reg2<T, 0, 2, R, E>(v, c, o.active); break;
+                case 3: if constexpr (KH > 3) pass_diag_reg2<T, 0, 3, R, E>(v, c, o.active); break;
+                case 6: if constexpr (KH > 2) pass_diag_reg2<T, 1, 2, R, E>(v, c, o.active); break;
+                case 7: if constexpr (KH > 3) pass_diag_reg2<T, 1, 3, R, E>(v, c, o.active); break;
+                case 11: if constexpr (KH > 3) pass_diag_reg2<T, 2, 3, R, E>(v, c, o.active); break;
+                default: break;
                 }
                 break; }
             default:

@@ -28,12 +28,11 @@ namespace {
 
 constexpr int kTileMinK = 10;   // 4 amplitudes per thread with 256 threads
 
-// The LDS tile pass is switched off: its first GPU runs (the 21-qubit QFT
-// subcircuit fixture, chunk_qbit 12) ended in an illegal memory access whose
-// cause is not yet shown.  Gate blocks run as per-gate sweeps, which compute
-// the same map (every gate of a block is tile-local).
+// SQ_TILE_PASSES=0 builds the blocked sweep without the LDS tile pass (gate
+// blocks then run as per-gate sweeps, the same map since every gate of a
+// block is tile-local).
 #ifndef SQ_TILE_PASSES
-#define SQ_TILE_PASSES 0
+#define SQ_TILE_PASSES 1
 #endif
 constexpr bool kTilePassesEnabled = SQ_TILE_PASSES != 0;
 constexpr int kTilePrefK = 10;  // 16 KiB (complex128) per workgroup: high occupancy

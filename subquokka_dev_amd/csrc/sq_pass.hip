@@ -40,11 +40,16 @@ void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vec
                     std::vector<cx<T>> &coef) {
     constexpr int VB = Traits<T>::VB;
     constexpr int S = VB + 6;
+    auto slot = [&](int t) {
+        const auto it = std::find(regs.begin(), regs.end(), t);
+        return it == regs.end() ? -1 : (int)(it - regs.begin());
+    };
+    auto push = [&](std::complex<double> z) { coef.push_back({(T)z.real(), (T)z.imag()}); };
     PassOp o{};
     o.coef = (uint32_t)coef.size();
-    const int nc = g.cls == OpClass::Dense ? 4 : (1 << g.k);
-    for (int i = 0; i < nc; i++) coef.push_back({(T)g.m[i].real(), (T)g.m[i].imag()});
+    o.b = -1;
     if (g.cls == OpClass::Dense) {
+        for (int i = 0; i < 4; i++) push(g.m[i]);
         const int t = g.t[0];
         if (t < VB) {
             o.kind = PO_DENSE_VEC;
@@ -53,12 +58,50 @@ void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vec
             o.a = t - VB;
         } else {
             o.kind = PO_DENSE_REG;
-            o.a = (int32_t)(std::find(regs.begin(), regs.end(), t) - regs.begin());
+            o.a = slot(t);
         }
+        ops.push_back(o);
+        return;
+    }
+    // diagonal: entry l = bit(t0) | bit(t1) << 1, t0 < t1
+    o.active = g.diag_active;
+    if (g.k == 1) {
+        const int t = g.t[0], j = slot(t);
+        if (j >= 0) {            // e[(register bit << 1) | other bit], no other bit
+            o.kind = PO_DIAG_REG1;
+            o.a = j;
+            push(g.m[0]); push(g.m[0]); push(g.m[1]); push(g.m[1]);
+        } else {
+            o.kind = PO_DIAG_FREE;
+            o.a = t;
+            o.c = t >= S;        // a tile bit: the same for the whole wave
+            push(g.m[0]); push(g.m[1]); push(g.m[0]); push(g.m[1]);
+        }
+        ops.push_back(o);
+        return;
+    }
+    const int t0 = g.t[0], t1 = g.t[1], j0 = slot(t0), j1 = slot(t1);
+    if (j0 >= 0 && j1 >= 0) {
+        o.kind = PO_DIAG_REG2;
+        o.a = j0;
+        o.b = j1;
+        for (int i = 0; i < 4; i++) push(g.m[i]);
+    } else if (j0 >= 0) {        // register bit is entry bit 0: e[(r << 1) | y] = d[r | y << 1]
+        o.kind = PO_DIAG_REG1;
+        o.a = j0;
+        o.b = t1;
+        push(g.m[0]); push(g.m[2]); push(g.m[1]); push(g.m[3]);
+    } else if (j1 >= 0) {        // register bit is entry bit 1: e = d
+        o.kind = PO_DIAG_REG1;
+        o.a = j1;
+        o.b = t0;
+        for (int i = 0; i < 4; i++) push(g.m[i]);
     } else {
-        o.kind = g.k == 1 ? PO_DIAG1 : PO_DIAG2;
-        o.a = g.t[0];
-        o.b = g.k == 2 ? g.t[1] : 0;
+        o.kind = PO_DIAG_FREE;
+        o.a = t0;
+        o.b = t1;
+        o.c = t0 >= S && t1 >= S;
+        for (int i = 0; i < 4; i++) push(g.m[i]);
     }
     ops.push_back(o);
 }

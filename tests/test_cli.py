@@ -101,8 +101,6 @@ def test_cli_state_files_hold_the_result(tmp_path, oracle, runner):
 
 
 @pytest.mark.gpu
-@pytest.mark.skip(reason="sq_apply_blocks faulted on the GPU on this fixture (illegal memory access, "
-                  "cause not shown); the CPU emulation build runs it instead (test_cpu_emul.py)")
 def test_cli_subcircuit_fixture_and_text_dump(tmp_path, oracle):
     n = 21
     text = open(os.path.join(GOLDEN, "ref_circuits", "sub_qft21.txt")).read()

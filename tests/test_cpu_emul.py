@@ -130,3 +130,43 @@ def test_out_of_core_staged_tiers_match_pinned(emul, tmp_path, dtype, n, w):
         assert np.array_equal(got, base), tier
     files = sorted(tmp_path.glob("state[0-9].bin"))
     assert len(files) == 4 and all(f.stat().st_size == (16 if dtype == "c128" else 8) << (n - 2) for f in files)
+
+
+def register_pass_circuit(n, n_gates, seed, first_dense):
+    """Dense one-qubit gates on qubits >= first_dense only (no lane exchange),
+    diagonal one- and two-qubit gates anywhere: every register-pass op kind
+    except the lane exchange, which the emulation does not model."""
+    import random as _random
+    from helpers import gate_line
+    rng = _random.Random(seed)
+    nrng = np.random.default_rng(seed)
+    lines = []
+    for _ in range(n_gates):
+        k = rng.choice(["H", "X", "Y", "U1", "RX", "RY", "Z", "P", "RZ", "CP", "RZZ", "CP", "RZZ"])
+        if k in ("CP", "RZZ"):
+            qs = sorted(rng.sample(range(n), 2))
+        elif k in ("Z", "P", "RZ"):
+            qs = [rng.randrange(n)]
+        else:
+            qs = [rng.randrange(first_dense, n)]
+        lines.append(gate_line(k, qs, rng, nrng))
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("dtype,first_dense", [("c128", 6), ("c64", 7), ("c64", 0)])
+@pytest.mark.parametrize("n", [11, 13])
+def test_register_pass_numerics(emul, tmp_path, oracle, dtype, n, first_dense):
+    # register passes computed exactly by the emulation (no wave-shared values):
+    # diagonal ops on lane, tile and register bits (free / one-register /
+    # two-register forms), dense ops on register bits and, for complex<float>
+    # with first_dense = 0, inside the vector -- against the oracle
+    if first_dense == 0:
+        text = register_pass_circuit(n, 250, 3 * n, 7).replace("\n", "\n", 1)
+        extra = "".join(f"H 0\nRX 0 {0.3 + 0.1 * i!r}\n" for i in range(5))
+        text = extra + text + extra
+    else:
+        text = register_pass_circuit(n, 250, 3 * n, first_dense)
+    got = run_case(emul, tmp_path, "plain", n, 4, dtype, text, seed=n)
+    want = oracle.run(sq.gen_state(n, n), n, text)
+    tol = 1e-12 if dtype == "c128" else 2e-6
+    assert np.max(np.abs(got - want)) <= tol

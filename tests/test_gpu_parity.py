@@ -49,16 +49,6 @@ def check(got, want, dtype, what=""):
         assert infidelity(got, want) <= INFID64, what
 
 
-# The subcircuit sweep (sq_apply_blocks) ended in an illegal memory access on
-# the GPU twice, on the 21-qubit QFT subcircuit fixture with the LDS tile pass
-# on.  Its cause is not shown: the CPU emulation build (tests/cpu_emul, ASan)
-# runs the same host code and kernel source with every access in bounds.  The
-# tile pass is switched off in the engine and every GPU case that runs through
-# sq_apply_blocks or the VSWAP_2_2 permutation kernel is skipped; the same
-# cases run in the CPU suite under the emulation (test_cpu_emul.py).
-BLOCKED_OFF = pytest.mark.skip(reason="sq_apply_blocks / VSWAP_2_2 kernel faulted on the GPU "
-                               "(illegal memory access, cause not shown); checked under CPU emulation instead")
-
 ONE_Q = ["H", "X", "Y", "Z", "P", "RX", "RY", "RZ", "U1"]
 TWO_Q = ["U2", "SWAP", "CP", "RZZ"]
 
@@ -109,7 +99,6 @@ def test_unsorted_symmetric_targets(oracle, dtype):
     check(run_gpu(n, text, s0, dtype), want, dtype, text)
 
 
-@BLOCKED_OFF
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("q", [(0, 1, 6, 9), (2, 5, 7, 11), (0, 3, 10, 11), (6, 8, 9, 12)])
 def test_vswap_2_2(oracle, dtype, q):
@@ -175,7 +164,6 @@ def test_config1_qft20_seeded_state(oracle, dtype):
     check(run_gpu(n, text, s0, dtype), want, dtype, "qft20")
 
 
-@BLOCKED_OFF
 @pytest.mark.parametrize("circ", ["qft", "qaoa", "h", "rx", "rzz"])
 def test_reference_sub_fixtures_blocked(oracle, circ):
     n = 21
@@ -252,7 +240,6 @@ def test_large_state_round_trip_property():
         eng.close()
 
 
-@BLOCKED_OFF
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("n,k", [(14, 5), (18, 4), (20, 6)])
 def test_planned_blocks_in_lds_tiles(oracle, dtype, n, k):
@@ -264,7 +251,6 @@ def test_planned_blocks_in_lds_tiles(oracle, dtype, n, k):
     check(run_gpu(n, plan, s0, dtype, sub=True), oracle.run(s0, n, text), dtype, f"tiles n={n}")
 
 
-@BLOCKED_OFF
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 def test_wide_block_is_split_into_tiles(oracle, dtype):
     # one block touching all 16 qubits: wider than any tile, split by the planner
@@ -275,7 +261,6 @@ def test_wide_block_is_split_into_tiles(oracle, dtype):
     check(run_gpu(n, sub, s0, dtype, sub=True), oracle.run(s0, n, text), dtype, "wide block")
 
 
-@BLOCKED_OFF
 @pytest.mark.parametrize("n,chunk", [(16, 10), (20, 12), (24, 12)])
 def test_finder_plan_vswap_blocks(oracle, n, chunk):
     # planner in the finder's format (blocks + VSWAP passes, order restored)
