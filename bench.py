@@ -58,6 +58,7 @@ def parse():
                          "sequence for rocprofv3 --pmc HBM-byte counts")
     ap.add_argument("--stream-dtype", default="c128")
     ap.add_argument("--resident-qbit", type=int, default=30)
+    ap.add_argument("--stream-layers", type=int, default=12)
     return ap.parse_args()
 
 
@@ -216,9 +217,10 @@ def workload_stream(args):
     """Out-of-core runner (DIO_Runner role, host RAM tier): the state lives in
     pinned host memory and is streamed through two HBM buffers of
     2^resident_qbit amplitudes, pass by pass; the same circuit then runs with
-    the state resident in HBM.  12 layers of the random-layer generator, seed 5."""
+    the state resident in HBM.  --stream-layers layers (default 12) of the
+    random-layer generator, seed 5."""
     import subquokka_dev_amd as sq
-    n, layers, seed, dtype = args.qubits, 12, 5, args.stream_dtype
+    n, layers, seed, dtype = args.qubits, args.stream_layers, 5, args.stream_dtype
     text = sq.gen_circuit("random", n, layers, seed=seed)
     per_layer = [l for l in text.splitlines() if l.strip()]
     ng = len(per_layer)
@@ -253,10 +255,10 @@ def workload_stream(args):
     zz_h, nrm_h = hbm.expect_zz_ring(), hbm.norm2()
     hbm.close()
     log(f"in HBM: {t_hbm:.2f} s; <ZZ> {zz_s:.12f} vs {zz_h:.12f}")
-    return line(f"{n}q random circuit, state in pinned host memory streamed through HBM: gates/s",
+    return line(f"{n}q random circuit, state outside HBM ({tier} host tier) streamed through HBM: gates/s",
                 ng / t_stream, "gates/s", 1, 0, t_stream * 1e3, "f64" if dtype == "c128" else "f32",
                 {"workload": f"random_layers_{n}q_{layers}l_seed{seed}_stream_{dtype}_w{args.resident_qbit}",
-                 "n_qubits": n, "layers": layers, "gates": ng, "state": f"{dtype} in pinned host memory"},
+                 "n_qubits": n, "layers": layers, "gates": ng, "state": f"{dtype} in {tier} host memory"},
                 passes=passes, host_tier=tier, host_link_gbs=round(link_bytes / t_stream / 1e9, 1),
                 host_link_bytes=link_bytes, in_hbm={"seconds": round(t_hbm, 3), "gates_s": round(ng / t_hbm, 2)},
                 stream_over_hbm_time=round(t_stream / t_hbm, 2),

@@ -147,7 +147,28 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
             v[r] = ld(s + off);
         }
 
-        for (int op = 0; op < a.n_ops; op++) {
+        // Diagonal ops without a register bit commute with every other diagonal
+        // op and with dense ops on register qubits: their factors collect in
+        // `ph` (one per lane and vector element) and reach the registers only
+        // before a lane or vector dense op and before the store.
+        cx<T> ph[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) ph[e] = {1, 0};
+        bool ph_set = false;   // uniform
+
+        // one flush site: before a lane or vector dense op, and after the last op
+        for (int op = 0; op <= a.n_ops; op++) {
+            const int kind = op < a.n_ops ? a.ops[op].kind : -1;
+            if (ph_set && (kind < 0 || kind == PO_DENSE_LANE || kind == PO_DENSE_VEC)) {
+#pragma unroll
+                for (int r = 0; r < R; r++)
+#pragma unroll
+                    for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], ph[e]);
+#pragma unroll
+                for (int e = 0; e < E; e++) ph[e] = {1, 0};
+                ph_set = false;
+            }
+            if (kind < 0) break;
             const PassOp o = a.ops[op];
             const cx<T> *c = a.coef + o.coef;
             switch (o.kind) {
@@ -196,22 +217,17 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                     if (!((o.active >> l) & 1)) break;
                     const cx<T> f = c[l];
 #pragma unroll
-                    for (int r = 0; r < R; r++)
-#pragma unroll
-                        for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], f);
+                    for (int e = 0; e < E; e++) ph[e] = cmul(ph[e], f);
+                    ph_set = true;
                     break;
                 }
-                cx<T> f[E];
 #pragma unroll
                 for (int e = 0; e < E; e++) {
                     const uint64_t idx = base + e;
                     const int x = (int)((idx >> o.a) & 1), y = o.b >= 0 ? (int)((idx >> o.b) & 1) : 0;
-                    f[e] = y ? (x ? c[3] : c[2]) : (x ? c[1] : c[0]);
+                    ph[e] = cmul(ph[e], y ? (x ? c[3] : c[2]) : (x ? c[1] : c[0]));
                 }
-#pragma unroll
-                for (int r = 0; r < R; r++)
-#pragma unroll
-                    for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], f[e]);
+                ph_set = true;
                 break; }
             case PO_DIAG_REG1: {
                 cx<T> ph0[E], ph1[E];

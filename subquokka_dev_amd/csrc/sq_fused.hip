@@ -19,6 +19,15 @@ using sq::LoweredGate;
 using sq::OpClass;
 using namespace sqk;
 
+// 16-column batches per wave and loop iteration (loads in flight), per
+// element type: complex<float> tiles are half the registers of complex<double>.
+#ifndef SQ_FUSED_U_F32
+#define SQ_FUSED_U_F32 2
+#endif
+#ifndef SQ_FUSED_U_F64
+#define SQ_FUSED_U_F64 2
+#endif
+
 namespace {
 
 struct FusedProgram {
@@ -104,7 +113,7 @@ int launch_apply(sq_engine *e, const cx<double> *u_dev, const std::vector<int> &
     a.u = u_dev;
     a.n_batches = 1ULL << (n - kFusedK - 4);
     for (int i = 0; i < kFusedK; i++) a.q[i] = q[i];
-    constexpr int U = 2;
+    constexpr int U = sizeof(T) == 4 ? SQ_FUSED_U_F32 : SQ_FUSED_U_F64;
     const unsigned grid = e->grid_flat(a.n_batches, 4 * U);
     int slot = e->stat_begin(nm, bytes);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fused5<T, U>), grid, 256, 0, e->stream, s, a);

@@ -356,16 +356,24 @@ int sq_engine::stream_norm_zz(double *v) {
         for (unsigned k = 0; k < t; k++)
             th.emplace_back([&, k] {
                 const uint64_t a = cnt * k / t, b = cnt * (k + 1) / t;
+                // two-level sums (4096-term blocks): the rounding error stays
+                // near one block's instead of growing with 2^n / threads terms
                 double x0 = 0, x1 = 0;
-                for (uint64_t j = a; j < b; j++) {
-                    double re, im;
-                    if (dtype == SQ_C128) { re = reinterpret_cast<double *>(p)[2 * j]; im = reinterpret_cast<double *>(p)[2 * j + 1]; }
-                    else { re = reinterpret_cast<float *>(p)[2 * j]; im = reinterpret_cast<float *>(p)[2 * j + 1]; }
-                    const double pr = re * re + im * im;
-                    const uint64_t i = a0 + j;
-                    const uint64_t rot = ((i >> 1) | (i << (n - 1))) & mask;
-                    x0 += pr;
-                    x1 += pr * (double)(n - 2 * __builtin_popcountll((i ^ rot) & mask));
+                for (uint64_t j0 = a; j0 < b; j0 += 4096) {
+                    double y0 = 0, y1 = 0;
+                    const uint64_t j1 = std::min<uint64_t>(b, j0 + 4096);
+                    for (uint64_t j = j0; j < j1; j++) {
+                        double re, im;
+                        if (dtype == SQ_C128) { re = reinterpret_cast<double *>(p)[2 * j]; im = reinterpret_cast<double *>(p)[2 * j + 1]; }
+                        else { re = reinterpret_cast<float *>(p)[2 * j]; im = reinterpret_cast<float *>(p)[2 * j + 1]; }
+                        const double pr = re * re + im * im;
+                        const uint64_t i = a0 + j;
+                        const uint64_t rot = ((i >> 1) | (i << (n - 1))) & mask;
+                        y0 += pr;
+                        y1 += pr * (double)(n - 2 * __builtin_popcountll((i ^ rot) & mask));
+                    }
+                    x0 += y0;
+                    x1 += y1;
                 }
                 s0[k] = x0; s1[k] = x1;
             });
