@@ -50,7 +50,7 @@ struct sq_engine {
     void *d_passprog = nullptr;     // register-pass program of the plain sweep
     size_t passprog_cap = 0;
     int sweep_mode = 0;             // 0: register passes, 1: one launch per gate
-    int pass_qbits = 4;             // register qubits per pass (1..5)
+    int pass_qbits = 4;             // register qubits per pass (1..4; set by sq_create)
 
     // out-of-core runner (resident_qbit > 0): state outside HBM
     int res_qbit = 0;

@@ -33,6 +33,7 @@ inline int __shfl_xor(int v, int, int = 64) { return v; }
 inline double __shfl_xor(double v, int, int = 64) { return v; }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline double __dadd_rn(double a, double b) { return a + b; }
+inline int __builtin_amdgcn_readfirstlane(int x) { return x; }
 inline double __dsub_rn(double a, double b) { return a - b; }
 inline double __dmul_rn(double a, double b) { return a * b; }
 
