@@ -57,7 +57,6 @@ template <typename T> struct PassArgs {
     uint64_t n_tiles;        // wave tiles = 2^(n - VB - 6 - KH)
     int n_ops;
     int hpos[kPassMaxKH];    // register qubits, ascending, all >= VB + 6
-    int lane_stride;         // 0 (see the op fetch in k_pass)
 };
 
 // One dense 2x2 on the register pairs (r, r | 2^J).
@@ -148,53 +147,9 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
             v[r] = ld(s + off);
         }
 
-        // Diagonal ops without a register bit commute with every other diagonal
-        // op and with dense ops on register qubits: their factors collect in
-        // `ph` (one per lane and vector element) and reach the registers only
-        // before a lane or vector dense op and before the store.
-        cx<T> ph[E];
-#pragma unroll
-        for (int e = 0; e < E; e++) ph[e] = {1, 0};
-        bool ph_set = false;   // uniform
-
-        // one flush site: before a lane or vector dense op, and after the last op
-        // The next op's record and coefficients are fetched one op ahead with
-        // vector loads: every lane reads the same record (offset lane *
-        // lane_stride, which is 0).  Vector loads complete in order, so the
-        // fetch overlaps the current op; scalar loads would make every op wait
-        // for all of them.  Fields that steer control flow go through
-        // readfirstlane.
-        const int ls = lane * a.lane_stride;
-        PassOp nx = a.ops[ls];   // the op array always holds at least one record
-        cx<T> nc0 = a.coef[nx.coef + ls], nc1 = a.coef[nx.coef + ls + 1], nc2 = a.coef[nx.coef + ls + 2],
-              nc3 = a.coef[nx.coef + ls + 3];
-
-        for (int op = 0; op <= a.n_ops; op++) {
-            const int kind = op < a.n_ops ? __builtin_amdgcn_readfirstlane(nx.kind) : -1;
-            if (ph_set && (kind < 0 || kind == PO_DENSE_LANE || kind == PO_DENSE_VEC)) {
-#pragma unroll
-                for (int r = 0; r < R; r++)
-#pragma unroll
-                    for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], ph[e]);
-#pragma unroll
-                for (int e = 0; e < E; e++) ph[e] = {1, 0};
-                ph_set = false;
-            }
-            if (kind < 0) break;
-            PassOp o;
-            o.kind = kind;
-            o.a = __builtin_amdgcn_readfirstlane(nx.a);
-            o.b = __builtin_amdgcn_readfirstlane(nx.b);
-            o.c = __builtin_amdgcn_readfirstlane(nx.c);
-            o.active = (uint32_t)__builtin_amdgcn_readfirstlane((int)nx.active);
-            const cx<T> c[4] = {nc0, nc1, nc2, nc3};
-            if (op + 1 < a.n_ops) {
-                nx = a.ops[op + 1 + ls];
-                nc0 = a.coef[nx.coef + ls];
-                nc1 = a.coef[nx.coef + ls + 1];
-                nc2 = a.coef[nx.coef + ls + 2];
-                nc3 = a.coef[nx.coef + ls + 3];
-            }
+        for (int op = 0; op < a.n_ops; op++) {
+            const PassOp o = a.ops[op];
+            const cx<T> *c = a.coef + o.coef;
             switch (o.kind) {
             case PO_DENSE_REG: {
                 const cx<T> m0 = c[0], m1 = c[1], m2 = c[2], m3 = c[3];
@@ -239,19 +194,24 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 if (o.c) {   // both bits are tile bits: one entry for the whole wave
                     const int l = (int)((base >> o.a) & 1) | (o.b >= 0 ? (int)((base >> o.b) & 1) << 1 : 0);
                     if (!((o.active >> l) & 1)) break;
-                    const cx<T> f = l & 2 ? (l & 1 ? c[3] : c[2]) : (l & 1 ? c[1] : c[0]);
+                    const cx<T> f = c[l];
 #pragma unroll
-                    for (int e = 0; e < E; e++) ph[e] = cmul(ph[e], f);
-                    ph_set = true;
+                    for (int r = 0; r < R; r++)
+#pragma unroll
+                        for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], f);
                     break;
                 }
+                cx<T> f[E];
 #pragma unroll
                 for (int e = 0; e < E; e++) {
                     const uint64_t idx = base + e;
                     const int x = (int)((idx >> o.a) & 1), y = o.b >= 0 ? (int)((idx >> o.b) & 1) : 0;
-                    ph[e] = cmul(ph[e], y ? (x ? c[3] : c[2]) : (x ? c[1] : c[0]));
+                    f[e] = y ? (x ? c[3] : c[2]) : (x ? c[1] : c[0]);
                 }
-                ph_set = true;
+#pragma unroll
+                for (int r = 0; r < R; r++)
+#pragma unroll
+                    for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], f[e]);
                 break; }
             case PO_DIAG_REG1: {
                 cx<T> ph0[E], ph1[E];

@@ -205,7 +205,6 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
         a.coef = dcoef;
         a.n_ops = l.n_ops;
         a.n_tiles = 1ULL << (n - S - kh);
-        a.lane_stride = 0;
         for (int j = 0; j < kh; j++) a.hpos[j] = l.hpos[j];
         int slot = e->stat_begin(nm, bytes, l.flops_per_amp * std::ldexp(1.0, n));
         int rc;
