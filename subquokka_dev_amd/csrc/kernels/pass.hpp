@@ -41,19 +41,45 @@ enum PassOpKind : int32_t {
 // which ignore the two low bits of their address components; byte fields
 // beside a dword field let the compiler address it from a base 2 bytes off
 // alignment, and the load then returns the byte fields instead.
-struct PassOp {
+// The four coefficients travel inside the record, so one scalar fetch brings
+// the whole op (the offset-then-load of a separate array costs a second
+// dependent round trip per op).
+template <typename T> struct alignas(sizeof(T) == 8 ? 128 : 64) PassOp {
     int32_t kind;
     int32_t a, b, c;
-    uint32_t coef;       // offset of the op's coefficients (4 for dense and diagonal)
     uint32_t active;     // diagonal: bitmask of entries that are not 1
-    int32_t pad[2];
+    int32_t pad[3];
+    cx<T> coef[4];
+};
+typedef int pass_v16i __attribute__((ext_vector_type(16)));
+
+// The whole record as dwords, read with one batch of aligned scalar loads.
+template <typename T> struct PassRec {
+    static constexpr int NV = sizeof(PassOp<T>) / 64;
+    pass_v16i w[NV];
+    __device__ explicit PassRec(const PassOp<T> *p) {
+        const pass_v16i *q = reinterpret_cast<const pass_v16i *>(p);
+#pragma unroll
+        for (int i = 0; i < NV; i++) w[i] = q[i];
+    }
+    __device__ int dw(int i) const { return w[i / 16][i % 16]; }
+    __device__ cx<T> coef(int j) const {
+        if constexpr (sizeof(T) == 8) {
+            const int b = 8 + 4 * j;
+            const uint64_t r = (uint64_t)(uint32_t)dw(b) | ((uint64_t)(uint32_t)dw(b + 1) << 32);
+            const uint64_t m = (uint64_t)(uint32_t)dw(b + 2) | ((uint64_t)(uint32_t)dw(b + 3) << 32);
+            return {__builtin_bit_cast(double, r), __builtin_bit_cast(double, m)};
+        } else {
+            const int b = 8 + 2 * j;
+            return {__builtin_bit_cast(float, dw(b)), __builtin_bit_cast(float, dw(b + 1))};
+        }
+    }
 };
 
 constexpr int kPassMaxKH = 4;   // five register qubits spill on gfx950
 
 template <typename T> struct PassArgs {
-    const PassOp *ops;
-    const cx<T> *coef;
+    const PassOp<T> *ops;
     uint64_t n_tiles;        // wave tiles = 2^(n - VB - 6 - KH)
     int n_ops;
     int hpos[kPassMaxKH];    // register qubits, ascending, all >= VB + 6
@@ -148,8 +174,11 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
         }
 
         for (int op = 0; op < a.n_ops; op++) {
-            const PassOp o = a.ops[op];
-            const cx<T> *c = a.coef + o.coef;
+            // every field is read up front: one batch of scalar loads, one wait
+            const PassRec<T> rec(a.ops + op);
+            struct { int kind, a, b, c; uint32_t active; } o = {rec.dw(0), rec.dw(1), rec.dw(2), rec.dw(3),
+                                                                (uint32_t)rec.dw(4)};
+            const cx<T> c[4] = {rec.coef(0), rec.coef(1), rec.coef(2), rec.coef(3)};
             switch (o.kind) {
             case PO_DENSE_REG: {
                 const cx<T> m0 = c[0], m1 = c[1], m2 = c[2], m3 = c[3];
@@ -194,7 +223,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 if (o.c) {   // both bits are tile bits: one entry for the whole wave
                     const int l = (int)((base >> o.a) & 1) | (o.b >= 0 ? (int)((base >> o.b) & 1) << 1 : 0);
                     if (!((o.active >> l) & 1)) break;
-                    const cx<T> f = c[l];
+                    const cx<T> f = l & 2 ? (l & 1 ? c[3] : c[2]) : (l & 1 ? c[1] : c[0]);
 #pragma unroll
                     for (int r = 0; r < R; r++)
 #pragma unroll

@@ -36,17 +36,16 @@ struct PassLaunch {
 };
 
 template <typename T>
-void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vector<PassOp> &ops,
-                    std::vector<cx<T>> &coef) {
+void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vector<PassOp<T>> &ops) {
     constexpr int VB = Traits<T>::VB;
     constexpr int S = VB + 6;
     auto slot = [&](int t) {
         const auto it = std::find(regs.begin(), regs.end(), t);
         return it == regs.end() ? -1 : (int)(it - regs.begin());
     };
-    auto push = [&](std::complex<double> z) { coef.push_back({(T)z.real(), (T)z.imag()}); };
-    PassOp o{};
-    o.coef = (uint32_t)coef.size();
+    PassOp<T> o{};
+    int nc = 0;
+    auto push = [&](std::complex<double> z) { o.coef[nc++] = {(T)z.real(), (T)z.imag()}; };
     o.b = -1;
     if (g.cls == OpClass::Dense) {
         for (int i = 0; i < 4; i++) push(g.m[i]);
@@ -137,8 +136,7 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
     const auto steps = sq::plan_register_passes(gates, mergeable, anywhere, n, S, kh);
 
     std::vector<PassLaunch> launches;
-    std::vector<PassOp> ops;
-    std::vector<cx<T>> coef;
+    std::vector<PassOp<T>> ops;
     for (const auto &st : steps) {
         double frac = 0;
         for (int gi : st.gates) frac += low[gi].pass_fraction;
@@ -159,7 +157,7 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
         l.merged = true;
         l.op_begin = (int)ops.size();
         for (int gi : st.gates) {
-            encode_pass_op<T>(low[gi], regs, ops, coef);
+            encode_pass_op<T>(low[gi], regs, ops);
             // dense 2x2 row: two complex products and a sum; diagonal: one product
             l.flops_per_amp += low[gi].cls == OpClass::Dense ? 14.0 : 6.0;
         }
@@ -169,8 +167,7 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
     }
 
     // upload the program once; earlier launches may still read the old one
-    const size_t ops_bytes = ((ops.size() * sizeof(PassOp) + 255) / 256) * 256;
-    const size_t need = ops_bytes + coef.size() * sizeof(cx<T>) + 256;
+    const size_t need = ops.size() * sizeof(PassOp<T>) + 256;
     if (!ops.empty()) {
         if (hipStreamSynchronize(e->stream) != hipSuccess) return e->fail(SQ_ERR_HIP, "stream sync");
         if (need > e->passprog_cap) {
@@ -182,13 +179,10 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
             e->passprog_cap = need * 2;
         }
         char *d = static_cast<char *>(e->d_passprog);
-        hipError_t he = hipMemcpy(d, ops.data(), ops.size() * sizeof(PassOp), hipMemcpyHostToDevice);
-        if (he == hipSuccess)
-            he = hipMemcpy(d + ops_bytes, coef.data(), coef.size() * sizeof(cx<T>), hipMemcpyHostToDevice);
+        hipError_t he = hipMemcpy(d, ops.data(), ops.size() * sizeof(PassOp<T>), hipMemcpyHostToDevice);
         if (he != hipSuccess) return e->hip_fail(he, "hipMemcpy(pass program)");
     }
-    const PassOp *dops = reinterpret_cast<const PassOp *>(e->d_passprog);
-    const cx<T> *dcoef = reinterpret_cast<const cx<T> *>(static_cast<char *>(e->d_passprog) + ops_bytes);
+    const PassOp<T> *dops = reinterpret_cast<const PassOp<T> *>(e->d_passprog);
 
     char nm[48];
     std::snprintf(nm, sizeof nm, "pass%d_%s", kh, e->dtype == SQ_C128 ? "c128" : "c64");
@@ -202,7 +196,6 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
         if (l.n_ops == 0) continue;   // the kernel reads its first op unconditionally
         PassArgs<T> a{};
         a.ops = dops + l.op_begin;
-        a.coef = dcoef;
         a.n_ops = l.n_ops;
         a.n_tiles = 1ULL << (n - S - kh);
         for (int j = 0; j < kh; j++) a.hpos[j] = l.hpos[j];
