@@ -28,6 +28,13 @@ HIP_FLAGS = [
 ]
 INCLUDES = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
+# Per-source flags.  sq_pass.hip: the register-pass op loop dispatches on
+# wave-uniform op records; left structurized, each uniform switch becomes flow
+# blocks whose merges copy the whole register-resident state once or twice per
+# op.  Skipping structurization of uniform regions keeps them as plain scalar
+# branches (divergent regions are still structurized).
+SOURCE_FLAGS = {"sq_pass.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions"]}
+
 HIP_SOURCES = ["sq_engine.hip", "sq_blocks.hip", "sq_fused.hip", "sq_adjoint.hip", "sq_stream.hip", "sq_pass.hip",
                "sq_sample.hip"]
 HOST_SOURCES = ["host/formats.cpp", "host/simulator.cpp", "host/planner.cpp"]
@@ -53,7 +60,7 @@ def build(verbose=True, jobs=8):
     procs, objs = [], []
     for src in HIP_SOURCES + HOST_SOURCES:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
-        flags = list(HIP_FLAGS)
+        flags = list(HIP_FLAGS) + SOURCE_FLAGS.get(src, [])
         if src.endswith(".cpp"):
             flags = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-x", "c++"]
             cmd = ["/usr/bin/g++"] + flags + INCLUDES + ["-c", os.path.join(CSRC, src), "-o", obj]

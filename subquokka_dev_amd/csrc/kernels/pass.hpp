@@ -85,6 +85,98 @@ template <typename T> struct PassArgs {
     int hpos[kPassMaxKH];    // register qubits, ascending, all >= VB + 6
 };
 
+// In-place updates.  The op loop carries the state in fixed registers, so
+// every op must leave each result in the register its input came from; a
+// result computed while its input is still needed lands elsewhere and costs
+// a copy per value at the end of the op.  Each output below is therefore a
+// partial sum of the other terms, formed while all inputs are live, plus one
+// final FMA that reads the input the output replaces, last.
+
+template <typename T> __device__ inline T fmaT(T a, T b, T c) {
+    if constexpr (sizeof(T) == 8) return __builtin_fma(a, b, c);
+    else return __builtin_fmaf(a, b, c);
+}
+
+// Loads, stores and lane shuffles field by field: a 16-byte copy would make
+// the loop carry each vector as one <2 x i64> value, and every op that
+// updates a field would then rebuild (and copy) the whole register tuple.
+template <typename T> __device__ inline void pass_ld(const cx<T> *p, Pack<T> &r) {
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
+    if constexpr (sizeof(T) == 8) {
+        typedef double v2d __attribute__((ext_vector_type(2)));
+        const v2d d = __builtin_bit_cast(v2d, v);
+        r.e[0].re = d.x;
+        r.e[0].im = d.y;
+    } else {
+        r.e[0].re = v.x; r.e[0].im = v.y; r.e[1].re = v.z; r.e[1].im = v.w;
+    }
+}
+template <typename T> __device__ inline void pass_st(cx<T> *p, const Pack<T> &r) {
+    v4f v;
+    if constexpr (sizeof(T) == 8) {
+        typedef double v2d __attribute__((ext_vector_type(2)));
+        v2d d;
+        d.x = r.e[0].re;
+        d.y = r.e[0].im;
+        v = __builtin_bit_cast(v4f, d);
+    } else {
+        v.x = r.e[0].re; v.y = r.e[0].im; v.z = r.e[1].re; v.w = r.e[1].im;
+    }
+    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(p));
+}
+template <typename T> __device__ inline T shfl_xor_T(T x, int mask) {
+    if constexpr (sizeof(T) == 8) {
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        v2i w = __builtin_bit_cast(v2i, x);
+        w.x = __shfl_xor(w.x, mask, 64);
+        w.y = __shfl_xor(w.y, mask, 64);
+        return __builtin_bit_cast(T, w);
+    } else {
+        return __builtin_bit_cast(T, __shfl_xor(__builtin_bit_cast(int, x), mask, 64));
+    }
+}
+
+// x = m x + p into x's own registers.  Left to itself the compiler emits the
+// two-address v_fmac, whose result goes to the addend's (the partial sum's)
+// registers, so every op would move the state out of its loop registers and
+// copy it back at the end of the op.
+template <typename T> __device__ inline void fma_into(T &x, const T m, const T p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(T) == 8) asm("v_fma_f64 %0, %1, %0, %2" : "+v"(x) : "v"(m), "v"(p));
+    else asm("v_fma_f32 %0, %1, %0, %2" : "+v"(x) : "v"(m), "v"(p));
+#else
+    x = fmaT(m, x, p);
+#endif
+}
+
+// x *= f: 2 temporaries, 4 FP64 ops.
+template <typename T> __device__ inline void ip_cmul(cx<T> &x, const cx<T> f) {
+    const T pr = -(x.im * f.im), pi = x.re * f.im;
+    fma_into(x.re, f.re, pr);
+    fma_into(x.im, f.re, pi);
+}
+
+// (x, y) <- (m0 x + m1 y, m2 x + m3 y): 4 temporaries, 16 FP64 ops.
+template <typename T>
+__device__ inline void ip_pair(cx<T> &x, cx<T> &y, const cx<T> m0, const cx<T> m1, const cx<T> m2, const cx<T> m3) {
+    const T pxr = fmaT(m1.re, y.re, fmaT(-m1.im, y.im, -(m0.im * x.im)));
+    const T pxi = fmaT(m1.re, y.im, fmaT(m1.im, y.re, m0.im * x.re));
+    const T pyr = fmaT(m2.re, x.re, fmaT(-m2.im, x.im, -(m3.im * y.im)));
+    const T pyi = fmaT(m2.re, x.im, fmaT(m2.im, x.re, m3.im * y.re));
+    fma_into(x.re, m0.re, pxr);
+    fma_into(x.im, m0.re, pxi);
+    fma_into(y.re, m3.re, pyr);
+    fma_into(y.im, m3.re, pyi);
+}
+
+// x <- ms x + mp p (p: the partner's value, already in other registers).
+template <typename T> __device__ inline void ip_lane(cx<T> &x, const cx<T> p, const cx<T> ms, const cx<T> mp) {
+    const T pr = fmaT(mp.re, p.re, fmaT(-mp.im, p.im, -(ms.im * x.im)));
+    const T pi = fmaT(mp.re, p.im, fmaT(mp.im, p.re, ms.im * x.re));
+    fma_into(x.re, ms.re, pr);
+    fma_into(x.im, ms.re, pi);
+}
+
 // One dense 2x2 on the register pairs (r, r | 2^J).
 template <typename T, int J, int R, int E>
 __device__ inline void pass_dense_reg(Pack<T> (&v)[R], const cx<T> m0, const cx<T> m1, const cx<T> m2, const cx<T> m3) {
@@ -93,16 +185,7 @@ __device__ inline void pass_dense_reg(Pack<T> (&v)[R], const cx<T> m0, const cx<
         if ((r >> J) & 1) continue;
         const int q = r | (1 << J);
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-            const cx<T> x = v[r].e[e], y = v[q].e[e];
-            cx<T> o0 = {0, 0}, o1 = {0, 0};
-            o0 = cfma(o0, m0, x);
-            o0 = cfma(o0, m1, y);
-            o1 = cfma(o1, m2, x);
-            o1 = cfma(o1, m3, y);
-            v[r].e[e] = o0;
-            v[q].e[e] = o1;
-        }
+        for (int e = 0; e < E; e++) ip_pair(v[r].e[e], v[q].e[e], m0, m1, m2, m3);
     }
 }
 
@@ -114,9 +197,11 @@ __device__ inline void pass_dense_lane(Pack<T> (&v)[R], bool hi, const cx<T> *c)
     const cx<T> ms = hi ? c[3] : c[0], mp = hi ? c[2] : c[1];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const Pack<T> p = shfl_xor_pack(v[r], MASK);
 #pragma unroll
-        for (int e = 0; e < E; e++) v[r].e[e] = cfma(cmul(ms, v[r].e[e]), mp, p.e[e]);
+        for (int e = 0; e < E; e++) {
+            const cx<T> p = {shfl_xor_T(v[r].e[e].re, MASK), shfl_xor_T(v[r].e[e].im, MASK)};
+            ip_lane(v[r].e[e], p, ms, mp);
+        }
     }
 }
 
@@ -127,7 +212,7 @@ __device__ inline void pass_diag_reg1(Pack<T> (&v)[R], const cx<T> (&ph0)[E], co
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], ((r >> J) & 1) ? ph1[e] : ph0[e]);
+        for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], ((r >> J) & 1) ? ph1[e] : ph0[e]);
 }
 
 // Diagonal on register slots J0 < J1: the factor of register r is d[l(r)],
@@ -140,7 +225,7 @@ __device__ inline void pass_diag_reg2(Pack<T> (&v)[R], const cx<T> *d, uint32_t 
         if (!((active >> l) & 1)) continue;
         const cx<T> f = d[l];
 #pragma unroll
-        for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], f);
+        for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], f);
     }
 }
 
@@ -151,7 +236,10 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
     constexpr int E = 1 << VB;
     constexpr int R = 1 << KH;
     const int lane = threadIdx.x & 63;
-    const uint64_t wave = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    // the wave index is uniform; readfirstlane lets the compiler know, so the
+    // tile loop and every branch on tile bits are scalar branches, not
+    // divergent regions that merge the state registers under exec masks
+    const uint64_t wave = (uint64_t)blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t n_waves = (uint64_t)gridDim.x * (kBlock / 64);
 
     uint64_t hb[KH];
@@ -159,10 +247,10 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
     for (int j = 0; j < KH; j++) hb[j] = 1ULL << a.hpos[j];
 
     for (uint64_t t = wave; t < a.n_tiles; t += n_waves) {
-        uint64_t base = t << S;
+        uint64_t tb = t << S;   // tile bits only (uniform)
 #pragma unroll
-        for (int j = 0; j < KH; j++) base = insert_zero(base, a.hpos[j]);
-        base += (uint64_t)lane << VB;
+        for (int j = 0; j < KH; j++) tb = insert_zero(tb, a.hpos[j]);
+        const uint64_t base = tb + ((uint64_t)lane << VB);
 
         Pack<T> v[R];
 #pragma unroll
@@ -170,7 +258,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
             uint64_t off = base;
 #pragma unroll
             for (int j = 0; j < KH; j++) if ((r >> j) & 1) off += hb[j];
-            v[r] = ld(s + off);
+            pass_ld(s + off, v[r]);
         }
 
         for (int op = 0; op < a.n_ops; op++) {
@@ -206,28 +294,19 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 if constexpr (E == 2) {
                     const cx<T> m0 = c[0], m1 = c[1], m2 = c[2], m3 = c[3];
 #pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        const cx<T> x = v[r].e[0], y = v[r].e[1];
-                        cx<T> o0 = {0, 0}, o1 = {0, 0};
-                        o0 = cfma(o0, m0, x);
-                        o0 = cfma(o0, m1, y);
-                        o1 = cfma(o1, m2, x);
-                        o1 = cfma(o1, m3, y);
-                        v[r].e[0] = o0;
-                        v[r].e[1] = o1;
-                    }
+                    for (int r = 0; r < R; r++) ip_pair(v[r].e[0], v[r].e[1], m0, m1, m2, m3);
                 }
                 break; }
             case PO_DIAG_FREE: {
                 // the factor depends only on lane, tile and vector bits: one per lane (and element)
                 if (o.c) {   // both bits are tile bits: one entry for the whole wave
-                    const int l = (int)((base >> o.a) & 1) | (o.b >= 0 ? (int)((base >> o.b) & 1) << 1 : 0);
+                    const int l = (int)((tb >> o.a) & 1) | (o.b >= 0 ? (int)((tb >> o.b) & 1) << 1 : 0);
                     if (!((o.active >> l) & 1)) break;
                     const cx<T> f = l & 2 ? (l & 1 ? c[3] : c[2]) : (l & 1 ? c[1] : c[0]);
 #pragma unroll
                     for (int r = 0; r < R; r++)
 #pragma unroll
-                        for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], f);
+                        for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], f);
                     break;
                 }
                 cx<T> f[E];
@@ -240,7 +319,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
 #pragma unroll
                 for (int r = 0; r < R; r++)
 #pragma unroll
-                    for (int e = 0; e < E; e++) v[r].e[e] = cmul(v[r].e[e], f[e]);
+                    for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], f[e]);
                 break; }
             case PO_DIAG_REG1: {
                 cx<T> ph0[E], ph1[E];
@@ -280,7 +359,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
             uint64_t off = base;
 #pragma unroll
             for (int j = 0; j < KH; j++) if ((r >> j) & 1) off += hb[j];
-            st(s + off, v[r]);
+            pass_st(s + off, v[r]);
         }
     }
 }

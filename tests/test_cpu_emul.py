@@ -25,7 +25,14 @@ pytestmark = pytest.mark.skipif(not os.path.exists(CLANG), reason="clang++ for t
 
 @pytest.fixture(scope="module")
 def emul():
-    subprocess.run(["bash", os.path.join(EMUL_DIR, "build.sh")], check=True)
+    # parallel test workers each build once; the lock and the rename keep one
+    # worker from running the binary while another rewrites it
+    import fcntl
+    with open(os.path.join(EMUL_DIR, ".build.lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        tmp = f"{EMUL}.tmp{os.getpid()}"
+        subprocess.run(["bash", os.path.join(EMUL_DIR, "build.sh")], check=True, env=dict(os.environ, EMUL_OUT=tmp))
+        os.replace(tmp, EMUL)
     return EMUL
 
 

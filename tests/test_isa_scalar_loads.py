@@ -27,10 +27,12 @@ UNALIGNED = re.compile(r"^\s*s_load_dword\w*\s+s\[?[0-9:]+\]?,\s*s\[[0-9:]+\],\s
 
 
 def device_asm(src, out_dir):
+    # the same per-source flags as the library build
+    from subquokka_dev_amd.build import SOURCE_FLAGS
     out = os.path.join(out_dir, src + ".s")
     cmd = [HIPCC, "-std=c++17", "-O3", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
            "-packed-fp32-ops", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "--cuda-device-only", "-S",
-           "-o", out, os.path.join(CSRC, src)]
+           "-o", out, os.path.join(CSRC, src)] + SOURCE_FLAGS.get(src, [])
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
     return out
