@@ -102,7 +102,7 @@ typedef struct sq_config {
     int32_t sweep_mode;       /* plain sweep: 0 = register passes (a run of one-qubit
                                  and diagonal gates per read/write of the state),
                                  1 = one launch per gate */
-    int32_t pass_qbits;       /* register qubits per pass, 1..4 (0: 4) */
+    int32_t pass_qbits;       /* register qubits per pass, 1..5 (0: 5 for SQ_C128, 4 for SQ_C64) */
     int32_t host_tier;        /* out-of-core runner: SQ_HOST_AUTO or SQ_HOST_PAGEABLE */
 } sq_config;
 

@@ -321,7 +321,7 @@ extern "C" int sq_plan(const char *circuit_text, int32_t n, int32_t mode, int32_
     } else if (mode == 2) {
         // register passes of the complex128 plain sweep: lane qubits 0-5 plus
         // at most param register qubits; other gates stand alone
-        if (param < 1 || param > 4) return SQ_ERR_ARG;
+        if (param < 1 || param > 5) return SQ_ERR_ARG;
         std::vector<char> mergeable(gates.size(), 0), anywhere(gates.size(), 0);
         for (size_t i = 0; i < gates.size(); i++) {
             sq::LoweredGate lg;

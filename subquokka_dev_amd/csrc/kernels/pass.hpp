@@ -76,7 +76,7 @@ template <typename T> struct PassRec {
     }
 };
 
-constexpr int kPassMaxKH = 4;   // five register qubits spill on gfx950
+constexpr int kPassMaxKH = 5;
 
 template <typename T> struct PassArgs {
     const PassOp<T> *ops;
@@ -275,6 +275,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 case 1: if constexpr (KH > 1) pass_dense_reg<T, 1, R, E>(v, m0, m1, m2, m3); break;
                 case 2: if constexpr (KH > 2) pass_dense_reg<T, 2, R, E>(v, m0, m1, m2, m3); break;
                 case 3: if constexpr (KH > 3) pass_dense_reg<T, 3, R, E>(v, m0, m1, m2, m3); break;
+                case 4: if constexpr (KH > 4) pass_dense_reg<T, 4, R, E>(v, m0, m1, m2, m3); break;
                 default: break;
                 }
                 break; }
@@ -335,17 +336,22 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 case 1: if constexpr (KH > 1) pass_diag_reg1<T, 1, R, E>(v, ph0, ph1); break;
                 case 2: if constexpr (KH > 2) pass_diag_reg1<T, 2, R, E>(v, ph0, ph1); break;
                 case 3: if constexpr (KH > 3) pass_diag_reg1<T, 3, R, E>(v, ph0, ph1); break;
+                case 4: if constexpr (KH > 4) pass_diag_reg1<T, 4, R, E>(v, ph0, ph1); break;
                 default: break;
                 }
                 break; }
             case PO_DIAG_REG2: {
-                switch (o.a * 4 + o.b) {
+                switch (o.a * 8 + o.b) {
                 case 1: if constexpr (KH > 1) pass_diag_reg2<T, 0, 1, R, E>(v, c, o.active); break;
                 case 2: if constexpr (KH > 2) pass_diag_reg2<T, 0, 2, R, E>(v, c, o.active); break;
                 case 3: if constexpr (KH > 3) pass_diag_reg2<T, 0, 3, R, E>(v, c, o.active); break;
-                case 6: if constexpr (KH > 2) pass_diag_reg2<T, 1, 2, R, E>(v, c, o.active); break;
-                case 7: if constexpr (KH > 3) pass_diag_reg2<T, 1, 3, R, E>(v, c, o.active); break;
-                case 11: if constexpr (KH > 3) pass_diag_reg2<T, 2, 3, R, E>(v, c, o.active); break;
+                case 4: if constexpr (KH > 4) pass_diag_reg2<T, 0, 4, R, E>(v, c, o.active); break;
+                case 10: if constexpr (KH > 2) pass_diag_reg2<T, 1, 2, R, E>(v, c, o.active); break;
+                case 11: if constexpr (KH > 3) pass_diag_reg2<T, 1, 3, R, E>(v, c, o.active); break;
+                case 12: if constexpr (KH > 4) pass_diag_reg2<T, 1, 4, R, E>(v, c, o.active); break;
+                case 19: if constexpr (KH > 3) pass_diag_reg2<T, 2, 3, R, E>(v, c, o.active); break;
+                case 20: if constexpr (KH > 4) pass_diag_reg2<T, 2, 4, R, E>(v, c, o.active); break;
+                case 28: if constexpr (KH > 4) pass_diag_reg2<T, 3, 4, R, E>(v, c, o.active); break;
                 default: break;
                 }
                 break; }

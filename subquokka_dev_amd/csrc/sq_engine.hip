@@ -276,7 +276,7 @@ int create_engine(const sq_config *cfg, const char *const *files, int n_files, b
     if (cfg->total_qbit < 1 || cfg->total_qbit > 36) return SQ_ERR_ARG;
     if (cfg->dtype != SQ_C128 && cfg->dtype != SQ_C64) return SQ_ERR_ARG;
     if (cfg->chunk_qbit < 0 || cfg->chunk_qbit > cfg->total_qbit) return SQ_ERR_ARG;
-    if (cfg->sweep_mode < 0 || cfg->sweep_mode > 1 || cfg->pass_qbits < 0 || cfg->pass_qbits > 4) return SQ_ERR_ARG;
+    if (cfg->sweep_mode < 0 || cfg->sweep_mode > 1 || cfg->pass_qbits < 0 || cfg->pass_qbits > 5) return SQ_ERR_ARG;
     if (cfg->host_tier != SQ_HOST_AUTO && cfg->host_tier != SQ_HOST_PAGEABLE) return SQ_ERR_ARG;
     // a resident chunk at least as large as the state means the state fits in HBM
     const int resident = cfg->resident_qbit >= cfg->total_qbit ? 0 : cfg->resident_qbit;
@@ -296,7 +296,7 @@ int create_engine(const sq_config *cfg, const char *const *files, int n_files, b
     e->sweep_mode = cfg->sweep_mode;
     // register qubits per pass: 4 for complex<double>; 3 for complex<float>,
     // whose 4-qubit pass exceeds the register file at two waves per SIMD
-    e->pass_qbits = cfg->pass_qbits > 0 ? cfg->pass_qbits : 4;
+    e->pass_qbits = cfg->pass_qbits > 0 ? cfg->pass_qbits : (cfg->dtype == SQ_C64 ? 4 : 5);
     if (hipSetDevice(e->device) != hipSuccess) { delete e; return SQ_ERR_HIP; }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { delete e; return SQ_ERR_HIP; }
     if (resident != 0) {

@@ -32,7 +32,7 @@ struct PassLaunch {
     int gate = -1;                 // single-gate step
     int op_begin = 0, n_ops = 0;   // merged step
     double flops_per_amp = 0;      // algorithmic real flops per amplitude
-    int hpos[kPassMaxKH] = {0, 0, 0, 0};
+    int hpos[kPassMaxKH] = {};
 };
 
 template <typename T>
@@ -206,6 +206,7 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
         case 2: rc = launch_pass<T, 2>(e, a); break;
         case 3: rc = launch_pass<T, 3>(e, a); break;
         case 4: rc = launch_pass<T, 4>(e, a); break;
+        case 5: rc = launch_pass<T, 5>(e, a); break;
         default: rc = e->fail(SQ_ERR_ARG, "internal: bad pass width");
         }
         e->stat_end(slot);

@@ -36,13 +36,15 @@ def emul():
     return EMUL
 
 
-def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None, tier=None):
+def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None, tier=None, pass_qbits=None):
     f = tmp_path / "c.txt"
     f.write_text(text)
     out = tmp_path / f"state_{tier or 'default'}.bin"
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
     if seed is not None:
         env["EMUL_INIT_SEED"] = str(seed)
+    if pass_qbits is not None:
+        env["EMUL_PASS_QBITS"] = str(pass_qbits)
     if tier is not None:
         env["EMUL_HOST_TIER"] = tier
         env["EMUL_FILES_DIR"] = str(tmp_path)
@@ -160,9 +162,10 @@ def register_pass_circuit(n, n_gates, seed, first_dense):
     return "\n".join(lines) + "\n"
 
 
+@pytest.mark.parametrize("kh", [4, 5])
 @pytest.mark.parametrize("dtype,first_dense", [("c128", 6), ("c64", 7), ("c64", 0)])
 @pytest.mark.parametrize("n", [11, 13])
-def test_register_pass_numerics(emul, tmp_path, oracle, dtype, n, first_dense):
+def test_register_pass_numerics(emul, tmp_path, oracle, dtype, n, first_dense, kh):
     # register passes computed exactly by the emulation (no wave-shared values):
     # diagonal ops on lane, tile and register bits (free / one-register /
     # two-register forms), dense ops on register bits and, for complex<float>
@@ -173,7 +176,7 @@ def test_register_pass_numerics(emul, tmp_path, oracle, dtype, n, first_dense):
         text = extra + text + extra
     else:
         text = register_pass_circuit(n, 250, 3 * n, first_dense)
-    got = run_case(emul, tmp_path, "plain", n, 4, dtype, text, seed=n)
+    got = run_case(emul, tmp_path, "plain", n, 4, dtype, text, seed=n, pass_qbits=kh)
     want = oracle.run(sq.gen_state(n, n), n, text)
     tol = 1e-12 if dtype == "c128" else 2e-6
     assert np.max(np.abs(got - want)) <= tol

@@ -120,7 +120,7 @@ def test_random_circuit(oracle, dtype, n):
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-@pytest.mark.parametrize("kh", [1, 2, 3, 4])
+@pytest.mark.parametrize("kh", [1, 2, 3, 4, 5])
 def test_register_passes_match_one_launch_per_gate(oracle, dtype, kh):
     # the plain sweep as register passes (kh register qubits) against the same
     # gates one launch each, and both against the oracle

@@ -97,7 +97,7 @@ DIAG = {"Z", "P", "RZ", "CP", "RZZ"}
 DENSE1 = {"H", "X", "Y", "U1", "RX", "RY"}
 
 
-@pytest.mark.parametrize("n,kh,seed", [(10, 1, 1), (11, 2, 2), (13, 4, 3), (16, 4, 4), (16, 3, 5)])
+@pytest.mark.parametrize("n,kh,seed", [(10, 1, 1), (11, 2, 2), (13, 4, 3), (16, 4, 4), (16, 3, 5), (16, 5, 6), (12, 5, 7)])
 def test_register_pass_schedule(oracle, n, kh, seed):
     # the plain sweep's schedule (sq_plan mode 2, complex128 lanes = qubits 0-5):
     #   * a step of several gates holds only one-qubit dense and diagonal gates,
