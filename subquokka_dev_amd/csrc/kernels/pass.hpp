@@ -124,15 +124,62 @@ template <typename T> __device__ inline void pass_st(cx<T> *p, const Pack<T> &r)
     }
     __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(p));
 }
-template <typename T> __device__ inline T shfl_xor_T(T x, int mask) {
+// Cross-lane exchanges without the LDS.
+// Lane bits 0 and 1: one DPP quad_perm per dword.  Lane bits 2 and 3: two DPP
+// moves that both read the source -- row_shl by 4 (8) written only to the
+// 4-lane banks whose lane bit is 0, then row_shr written to the others -- so
+// no DPP reads a register a VALU op has just written (that would need wait
+// states).  Lane bits 4 and 5: v_permlane16_swap / v_permlane32_swap, which
+// exchange the odd 16-lane rows (upper 32 lanes) of one register with the
+// even rows (lower 32 lanes) of another; applied to a register pair that
+// differs in register bit 0, this swaps lane bit 4 (5) with register bit 0,
+// and a second application swaps them back.
+template <int MASK> __device__ inline int lane_xor_dword(int x) {
+    static_assert(MASK == 1 || MASK == 2 || MASK == 4 || MASK == 8, "DPP exchange covers lane bits 0-3");
+    if constexpr (MASK == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    else if constexpr (MASK == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    else if constexpr (MASK == 4) {
+        const int lo = __builtin_amdgcn_mov_dpp(x, 0x104, 0xF, 0x5, false);      // row_shl:4 -> banks 0, 2
+        return __builtin_amdgcn_update_dpp(lo, x, 0x114, 0xF, 0xA, false);       // row_shr:4 -> banks 1, 3
+    } else {
+        const int lo = __builtin_amdgcn_mov_dpp(x, 0x108, 0xF, 0x3, false);      // row_shl:8 -> banks 0, 1
+        return __builtin_amdgcn_update_dpp(lo, x, 0x118, 0xF, 0xC, false);       // row_shr:8 -> banks 2, 3
+    }
+}
+template <int MASK, typename T> __device__ inline T lane_xor(T x) {
     if constexpr (sizeof(T) == 8) {
         typedef int v2i __attribute__((ext_vector_type(2)));
         v2i w = __builtin_bit_cast(v2i, x);
-        w.x = __shfl_xor(w.x, mask, 64);
-        w.y = __shfl_xor(w.y, mask, 64);
+        w.x = lane_xor_dword<MASK>(w.x);
+        w.y = lane_xor_dword<MASK>(w.y);
         return __builtin_bit_cast(T, w);
     } else {
-        return __builtin_bit_cast(T, __shfl_xor(__builtin_bit_cast(int, x), mask, 64));
+        return __builtin_bit_cast(T, lane_xor_dword<MASK>(__builtin_bit_cast(int, x)));
+    }
+}
+template <int MASK> __device__ inline void permlane_swap_dword(int &a, int &b) {
+    static_assert(MASK == 16 || MASK == 32, "permlane swaps cover lane bits 4 and 5");
+    if constexpr (MASK == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+        a = r[0]; b = r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+        a = r[0]; b = r[1];
+    }
+}
+template <int MASK, typename T> __device__ inline void permlane_swap(T &a, T &b) {
+    if constexpr (sizeof(T) == 8) {
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        v2i x = __builtin_bit_cast(v2i, a), y = __builtin_bit_cast(v2i, b);
+        int x0 = x.x, x1 = x.y, y0 = y.x, y1 = y.y;
+        permlane_swap_dword<MASK>(x0, y0);
+        permlane_swap_dword<MASK>(x1, y1);
+        x.x = x0; x.y = x1; y.x = y0; y.y = y1;
+        a = __builtin_bit_cast(T, x); b = __builtin_bit_cast(T, y);
+    } else {
+        int x = __builtin_bit_cast(int, a), y = __builtin_bit_cast(int, b);
+        permlane_swap_dword<MASK>(x, y);
+        a = __builtin_bit_cast(T, x); b = __builtin_bit_cast(T, y);
     }
 }
 
@@ -147,6 +194,21 @@ template <typename T> __device__ inline void fma_into(T &x, const T m, const T p
 #else
     x = fmaT(m, x, p);
 #endif
+}
+
+// The four coefficients of an op as separate values.  Selections between them
+// are made per component (v_cndmask on values): a select between array
+// elements would select addresses, and the array would then live in memory
+// (LDS or scratch) instead of registers.
+template <typename T> struct Coef4 {
+    cx<T> c0, c1, c2, c3;
+};
+template <typename T> __device__ inline cx<T> csel(bool p, const cx<T> a, const cx<T> b) {
+    return {p ? a.re : b.re, p ? a.im : b.im};
+}
+// entry (y << 1) | x
+template <typename T> __device__ inline cx<T> csel4(bool y, bool x, const Coef4<T> &k) {
+    return csel(y, csel(x, k.c3, k.c2), csel(x, k.c1, k.c0));
 }
 
 // x *= f: 2 temporaries, 4 FP64 ops.
@@ -189,18 +251,37 @@ __device__ inline void pass_dense_reg(Pack<T> (&v)[R], const cx<T> m0, const cx<
     }
 }
 
-// Dense 2x2 on lane bit MASK: a lane whose bit is 0 holds x0 and gets
-// m0 x0 + m1 x1; its partner holds x1 and gets m3 x1 + m2 x0.  The per-lane
-// coefficients are chosen once per op; the shuffle mask is a constant.
+// Dense 2x2 on lane bit MASK.
+// Lane bits 0-3: a lane whose bit is 0 holds x0 and gets m0 x0 + m1 x1; its
+// partner holds x1 and gets m3 x1 + m2 x0.  The per-lane coefficients are
+// chosen once per op; the partner's value comes from a DPP permutation.
+// Lane bits 4-5: each register pair (r, r | 1) swaps the lane bit with register
+// bit 0, takes the gate as a register-pair op, and swaps back.
 template <typename T, int MASK, int R, int E>
-__device__ inline void pass_dense_lane(Pack<T> (&v)[R], bool hi, const cx<T> *c) {
-    const cx<T> ms = hi ? c[3] : c[0], mp = hi ? c[2] : c[1];
+__device__ inline void pass_dense_lane(Pack<T> (&v)[R], bool hi, const Coef4<T> k) {
+    if constexpr (MASK <= 8) {
+        const cx<T> ms = csel(hi, k.c3, k.c0), mp = csel(hi, k.c2, k.c1);
 #pragma unroll
-    for (int r = 0; r < R; r++) {
+        for (int r = 0; r < R; r++) {
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-            const cx<T> p = {shfl_xor_T(v[r].e[e].re, MASK), shfl_xor_T(v[r].e[e].im, MASK)};
-            ip_lane(v[r].e[e], p, ms, mp);
+            for (int e = 0; e < E; e++) {
+                const cx<T> p = {lane_xor<MASK>(v[r].e[e].re), lane_xor<MASK>(v[r].e[e].im)};
+                ip_lane(v[r].e[e], p, ms, mp);
+            }
+        }
+    } else {
+        static_assert(R >= 2, "the swap needs a register bit");
+#pragma unroll
+        for (int r = 0; r < R; r += 2) {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                cx<T> &x = v[r].e[e], &y = v[r + 1].e[e];
+                permlane_swap<MASK>(x.re, y.re);
+                permlane_swap<MASK>(x.im, y.im);
+                ip_pair(x, y, k.c0, k.c1, k.c2, k.c3);
+                permlane_swap<MASK>(x.re, y.re);
+                permlane_swap<MASK>(x.im, y.im);
+            }
         }
     }
 }
@@ -218,12 +299,12 @@ __device__ inline void pass_diag_reg1(Pack<T> (&v)[R], const cx<T> (&ph0)[E], co
 // Diagonal on register slots J0 < J1: the factor of register r is d[l(r)],
 // known per r at compile time; entries equal to 1 are skipped.
 template <typename T, int J0, int J1, int R, int E>
-__device__ inline void pass_diag_reg2(Pack<T> (&v)[R], const cx<T> *d, uint32_t active) {
+__device__ inline void pass_diag_reg2(Pack<T> (&v)[R], const Coef4<T> d, uint32_t active) {
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int l = ((r >> J0) & 1) | (((r >> J1) & 1) << 1);
         if (!((active >> l) & 1)) continue;
-        const cx<T> f = d[l];
+        const cx<T> f = csel(l == 0, d.c0, csel(l == 1, d.c1, csel(l == 2, d.c2, d.c3)));   // l folds per r
 #pragma unroll
         for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], f);
     }
@@ -266,10 +347,10 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
             const PassRec<T> rec(a.ops + op);
             struct { int kind, a, b, c; uint32_t active; } o = {rec.dw(0), rec.dw(1), rec.dw(2), rec.dw(3),
                                                                 (uint32_t)rec.dw(4)};
-            const cx<T> c[4] = {rec.coef(0), rec.coef(1), rec.coef(2), rec.coef(3)};
+            const Coef4<T> c = {rec.coef(0), rec.coef(1), rec.coef(2), rec.coef(3)};
             switch (o.kind) {
             case PO_DENSE_REG: {
-                const cx<T> m0 = c[0], m1 = c[1], m2 = c[2], m3 = c[3];
+                const cx<T> m0 = c.c0, m1 = c.c1, m2 = c.c2, m3 = c.c3;
                 switch (o.a) {
                 case 0: pass_dense_reg<T, 0, R, E>(v, m0, m1, m2, m3); break;
                 case 1: if constexpr (KH > 1) pass_dense_reg<T, 1, R, E>(v, m0, m1, m2, m3); break;
@@ -293,7 +374,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 break; }
             case PO_DENSE_VEC: {
                 if constexpr (E == 2) {
-                    const cx<T> m0 = c[0], m1 = c[1], m2 = c[2], m3 = c[3];
+                    const cx<T> m0 = c.c0, m1 = c.c1, m2 = c.c2, m3 = c.c3;
 #pragma unroll
                     for (int r = 0; r < R; r++) ip_pair(v[r].e[0], v[r].e[1], m0, m1, m2, m3);
                 }
@@ -303,7 +384,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 if (o.c) {   // both bits are tile bits: one entry for the whole wave
                     const int l = (int)((tb >> o.a) & 1) | (o.b >= 0 ? (int)((tb >> o.b) & 1) << 1 : 0);
                     if (!((o.active >> l) & 1)) break;
-                    const cx<T> f = l & 2 ? (l & 1 ? c[3] : c[2]) : (l & 1 ? c[1] : c[0]);
+                    const cx<T> f = csel4((l & 2) != 0, (l & 1) != 0, c);
 #pragma unroll
                     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -315,7 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 for (int e = 0; e < E; e++) {
                     const uint64_t idx = base + e;
                     const int x = (int)((idx >> o.a) & 1), y = o.b >= 0 ? (int)((idx >> o.b) & 1) : 0;
-                    f[e] = y ? (x ? c[3] : c[2]) : (x ? c[1] : c[0]);
+                    f[e] = csel4(y != 0, x != 0, c);
                 }
 #pragma unroll
                 for (int r = 0; r < R; r++)
@@ -328,8 +409,8 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 for (int e = 0; e < E; e++) {
                     const uint64_t idx = base + e;
                     const int y = o.b >= 0 ? (int)((idx >> o.b) & 1) : 0;
-                    ph0[e] = y ? c[1] : c[0];
-                    ph1[e] = y ? c[3] : c[2];
+                    ph0[e] = csel(y != 0, c.c1, c.c0);
+                    ph1[e] = csel(y != 0, c.c3, c.c2);
                 }
                 switch (o.a) {
                 case 0: pass_diag_reg1<T, 0, R, E>(v, ph0, ph1); break;

@@ -34,6 +34,13 @@ inline double __shfl_xor(double v, int, int = 64) { return v; }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline double __dadd_rn(double a, double b) { return a + b; }
 inline int __builtin_amdgcn_readfirstlane(int x) { return x; }
+// Cross-lane permutations are not modelled (one lane runs at a time): the
+// value stays with its lane, like __shfl_xor above.
+inline int __builtin_amdgcn_update_dpp(int, int x, int, int, int, bool) { return x; }
+inline int __builtin_amdgcn_mov_dpp(int x, int, int, int, bool) { return x; }
+typedef unsigned sq_emul_u2 __attribute__((ext_vector_type(2)));
+inline sq_emul_u2 __builtin_amdgcn_permlane16_swap(unsigned a, unsigned b, bool, bool) { return sq_emul_u2{a, b}; }
+inline sq_emul_u2 __builtin_amdgcn_permlane32_swap(unsigned a, unsigned b, bool, bool) { return sq_emul_u2{a, b}; }
 inline double __dsub_rn(double a, double b) { return a - b; }
 inline double __dmul_rn(double a, double b) { return a * b; }
 

@@ -50,3 +50,24 @@ def test_no_scalar_load_at_unaligned_offset(tmp_path):
                 bad.append(f"{src}.s:{i + 1}: {line.strip()}")
     assert not bad, "\n".join(bad[:20])
     assert shutil.which(HIPCC) or os.path.exists(HIPCC)
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
+def test_register_pass_state_stays_in_registers(tmp_path):
+    # k_pass keeps a wave's 2^KH x 64 vectors and the op's coefficients in
+    # VGPRs; an array the compiler cannot split (e.g. one indexed through a
+    # selected address) moves to scratch or is promoted to LDS, and every op
+    # then pays a memory round trip.  Every instance must use neither.
+    path = device_asm("sq_pass.hip", str(tmp_path))
+    sizes, name = {}, None
+    for line in open(path):
+        t = line.split()
+        if len(t) >= 2 and t[0] == ".amdhsa_kernel":
+            name = t[1]
+        elif name and len(t) == 2 and t[0] in (".amdhsa_group_segment_fixed_size",
+                                               ".amdhsa_private_segment_fixed_size"):
+            sizes.setdefault(name, {})[t[0]] = int(t[1])
+    passes = {k: v for k, v in sizes.items() if "k_pass" in k}
+    assert len(passes) == 10, sorted(sizes)   # complex128 / complex64, 1-5 register qubits
+    bad = {k: v for k, v in passes.items() if any(v.values())}
+    assert not bad, bad
