@@ -33,7 +33,8 @@ enum PassOpKind : int32_t {
                          //   c = 1 when both bits are wave-uniform (tile bits), so identity
                          //   entries skip the whole op
     PO_DIAG_REG1 = 4,    // diagonal, one bit in register slot a, the other at global bit b (< 0: none);
-                         //   coefficients e[(register bit << 1) | other bit]
+                         //   coefficients e[(register bit << 1) | other bit]; c = 1 when the other
+                         //   bit is a tile bit or absent (wave-uniform)
     PO_DIAG_REG2 = 5,    // diagonal, both bits in register slots a < b
 };
 
@@ -47,7 +48,7 @@ enum PassOpKind : int32_t {
 template <typename T> struct alignas(sizeof(T) == 8 ? 128 : 64) PassOp {
     int32_t kind;
     int32_t a, b, c;
-    uint32_t active;     // diagonal: bitmask of entries that are not 1
+    uint32_t active;     // diagonal: bit k set when coefficient k is not 1
     int32_t pad[3];
     cx<T> coef[4];
 };
@@ -287,26 +288,41 @@ __device__ inline void pass_dense_lane(Pack<T> (&v)[R], bool hi, const Coef4<T> 
 }
 
 // Diagonal with a bit in register slot J: ph0 / ph1 are the lane's factors
-// for register bit 0 / 1 (per vector element).
+// for register bit 0 / 1 (per vector element).  A half whose factors are all
+// 1 (act0 / act1 false, wave-uniform) is skipped.
 template <typename T, int J, int R, int E>
-__device__ inline void pass_diag_reg1(Pack<T> (&v)[R], const cx<T> (&ph0)[E], const cx<T> (&ph1)[E]) {
+__device__ inline void pass_diag_reg1(Pack<T> (&v)[R], const cx<T> (&ph0)[E], const cx<T> (&ph1)[E], bool act0,
+                                      bool act1) {
+    if (act0) {
 #pragma unroll
-    for (int r = 0; r < R; r++)
+        for (int r = 0; r < R; r++)
+            if (!((r >> J) & 1))
 #pragma unroll
-        for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], ((r >> J) & 1) ? ph1[e] : ph0[e]);
+                for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], ph0[e]);
+    }
+    if (act1) {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if ((r >> J) & 1)
+#pragma unroll
+                for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], ph1[e]);
+    }
 }
 
 // Diagonal on register slots J0 < J1: the factor of register r is d[l(r)],
-// known per r at compile time; entries equal to 1 are skipped.
+// known per r at compile time.  One uniform branch per entry skips the
+// entries equal to 1 (a controlled phase touches a quarter of the registers).
 template <typename T, int J0, int J1, int R, int E>
 __device__ inline void pass_diag_reg2(Pack<T> (&v)[R], const Coef4<T> d, uint32_t active) {
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int l = ((r >> J0) & 1) | (((r >> J1) & 1) << 1);
+    for (int l = 0; l < 4; l++) {
         if (!((active >> l) & 1)) continue;
-        const cx<T> f = csel(l == 0, d.c0, csel(l == 1, d.c1, csel(l == 2, d.c2, d.c3)));   // l folds per r
+        const cx<T> f = csel(l == 0, d.c0, csel(l == 1, d.c1, csel(l == 2, d.c2, d.c3)));   // l folds
 #pragma unroll
-        for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], f);
+        for (int r = 0; r < R; r++)
+            if ((((r >> J0) & 1) | (((r >> J1) & 1) << 1)) == l)
+#pragma unroll
+                for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], f);
     }
 }
 
@@ -404,20 +420,35 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                     for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], f[e]);
                 break; }
             case PO_DIAG_REG1: {
+                // entries e[(register bit << 1) | y]; y is the other bit of the index
                 cx<T> ph0[E], ph1[E];
+                bool act0, act1;
+                if (o.c) {   // y is a tile bit (or absent): the same for the whole wave
+                    const int y = o.b >= 0 ? (int)((tb >> o.b) & 1) : 0;
+                    act0 = (o.active >> y) & 1;
+                    act1 = (o.active >> (2 + y)) & 1;
 #pragma unroll
-                for (int e = 0; e < E; e++) {
-                    const uint64_t idx = base + e;
-                    const int y = o.b >= 0 ? (int)((idx >> o.b) & 1) : 0;
-                    ph0[e] = csel(y != 0, c.c1, c.c0);
-                    ph1[e] = csel(y != 0, c.c3, c.c2);
+                    for (int e = 0; e < E; e++) {
+                        ph0[e] = csel(y != 0, c.c1, c.c0);
+                        ph1[e] = csel(y != 0, c.c3, c.c2);
+                    }
+                } else {     // y is a lane or vector bit
+                    act0 = (o.active & 3u) != 0;
+                    act1 = (o.active & 12u) != 0;
+#pragma unroll
+                    for (int e = 0; e < E; e++) {
+                        const uint64_t idx = base + e;
+                        const int y = (int)((idx >> o.b) & 1);
+                        ph0[e] = csel(y != 0, c.c1, c.c0);
+                        ph1[e] = csel(y != 0, c.c3, c.c2);
+                    }
                 }
                 switch (o.a) {
-                case 0: pass_diag_reg1<T, 0, R, E>(v, ph0, ph1); break;
-                case 1: if constexpr (KH > 1) pass_diag_reg1<T, 1, R, E>(v, ph0, ph1); break;
-                case 2: if constexpr (KH > 2) pass_diag_reg1<T, 2, R, E>(v, ph0, ph1); break;
-                case 3: if constexpr (KH > 3) pass_diag_reg1<T, 3, R, E>(v, ph0, ph1); break;
-                case 4: if constexpr (KH > 4) pass_diag_reg1<T, 4, R, E>(v, ph0, ph1); break;
+                case 0: pass_diag_reg1<T, 0, R, E>(v, ph0, ph1, act0, act1); break;
+                case 1: if constexpr (KH > 1) pass_diag_reg1<T, 1, R, E>(v, ph0, ph1, act0, act1); break;
+                case 2: if constexpr (KH > 2) pass_diag_reg1<T, 2, R, E>(v, ph0, ph1, act0, act1); break;
+                case 3: if constexpr (KH > 3) pass_diag_reg1<T, 3, R, E>(v, ph0, ph1, act0, act1); break;
+                case 4: if constexpr (KH > 4) pass_diag_reg1<T, 4, R, E>(v, ph0, ph1, act0, act1); break;
                 default: break;
                 }
                 break; }

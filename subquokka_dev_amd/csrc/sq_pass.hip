@@ -62,19 +62,29 @@ void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vec
         ops.push_back(o);
         return;
     }
-    // diagonal: entry l = bit(t0) | bit(t1) << 1, t0 < t1
-    o.active = g.diag_active;
+    // diagonal: entry l = bit(t0) | bit(t1) << 1, t0 < t1.  The op's entries are
+    // the gate's entries in the order the kernel reads them; bit k of active
+    // is set when entry k is not 1.
+    auto entries = [&](int s0, int s1, int s2, int s3) {
+        const int src[4] = {s0, s1, s2, s3};
+        o.active = 0;
+        for (int k = 0; k < 4; k++) {
+            push(g.m[src[k]]);
+            o.active |= ((g.diag_active >> src[k]) & 1u) << k;
+        }
+    };
     if (g.k == 1) {
         const int t = g.t[0], j = slot(t);
         if (j >= 0) {            // e[(register bit << 1) | other bit], no other bit
             o.kind = PO_DIAG_REG1;
             o.a = j;
-            push(g.m[0]); push(g.m[0]); push(g.m[1]); push(g.m[1]);
+            o.c = 1;             // no lane-dependent bit: factors per register half
+            entries(0, 0, 1, 1);
         } else {
             o.kind = PO_DIAG_FREE;
             o.a = t;
             o.c = t >= S;        // a tile bit: the same for the whole wave
-            push(g.m[0]); push(g.m[1]); push(g.m[0]); push(g.m[1]);
+            entries(0, 1, 0, 1);
         }
         ops.push_back(o);
         return;
@@ -84,23 +94,25 @@ void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vec
         o.kind = PO_DIAG_REG2;
         o.a = j0;
         o.b = j1;
-        for (int i = 0; i < 4; i++) push(g.m[i]);
+        entries(0, 1, 2, 3);
     } else if (j0 >= 0) {        // register bit is entry bit 0: e[(r << 1) | y] = d[r | y << 1]
         o.kind = PO_DIAG_REG1;
         o.a = j0;
         o.b = t1;
-        push(g.m[0]); push(g.m[2]); push(g.m[1]); push(g.m[3]);
+        o.c = t1 >= S;           // y a tile bit: one factor per register half for the wave
+        entries(0, 2, 1, 3);
     } else if (j1 >= 0) {        // register bit is entry bit 1: e = d
         o.kind = PO_DIAG_REG1;
         o.a = j1;
         o.b = t0;
-        for (int i = 0; i < 4; i++) push(g.m[i]);
+        o.c = t0 >= S;
+        entries(0, 1, 2, 3);
     } else {
         o.kind = PO_DIAG_FREE;
         o.a = t0;
         o.b = t1;
         o.c = t0 >= S && t1 >= S;
-        for (int i = 0; i < 4; i++) push(g.m[i]);
+        entries(0, 1, 2, 3);
     }
     ops.push_back(o);
 }
