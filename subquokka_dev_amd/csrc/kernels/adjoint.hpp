@@ -5,54 +5,64 @@
 // E = <psi|H|psi>, the reverse sweep holds psi_k (state after gate k) and
 // lambda_k = G_{k+1}^dag ... G_L^dag H psi_L and computes
 //     dE/dtheta_k = 2 Re <lambda_k| dG_k/dtheta |psi_{k-1}> = Im <lambda_k| P |psi_k>,
-// then un-applies G_k to both vectors.  One kernel does all three in a single
-// pass over psi and lambda; the inner product is accumulated in FP64.
+// then un-applies G_k to both vectors.  One kernel does this for both rotations
+// of a qubit in a single pass over psi and lambda; the inner products are
+// accumulated in FP64.
 #pragma once
 #include "common.hpp"
 #include "reduce.hpp"
 
 namespace sqk {
 
-struct AdjRotArgs {
+struct AdjRot2Args {
     uint64_t n_pairs;
     int q;
-    int axis;          // 0: Y, 1: Z
-    double c, s;       // cos(theta/2), sin(theta/2)
+    double cz, sz;     // cos, sin of theta/2 for RZ(q)
+    double cy, sy;     // cos, sin of theta/2 for RY(q)
 };
 
+// The two rotations of one qubit in one pass: the reverse sweep meets RZ(q)
+// and then RY(q) of a layer.  Rotations on other qubits commute with both
+// (and with their generators), so the sweep may take each qubit's pair
+// together.  Per amplitude pair: the RZ gradient term, un-apply RZ, the RY
+// gradient term from the updated values, un-apply RY -- the same arithmetic,
+// in the same order and precision, as one pass per rotation.  acc[0] is the RY
+// parameter's partial sum and acc[1] the RZ parameter's, written `stride`
+// apart.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_adj_rot(cx<T> *__restrict__ psi, cx<T> *__restrict__ lam,
-                                                    const AdjRotArgs a, double *__restrict__ part) {
-    double acc[1] = {0.0};
-    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    const T c = (T)a.c, s = (T)a.s;
-    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < a.n_pairs; p += stride) {
+__global__ void __launch_bounds__(kBlock) k_adj_rot2(cx<T> *__restrict__ psi, cx<T> *__restrict__ lam,
+                                                     const AdjRot2Args a, double *__restrict__ part, uint64_t stride) {
+    double acc[2] = {0.0, 0.0};
+    const uint64_t gstride = (uint64_t)gridDim.x * kBlock;
+    const T cz = (T)a.cz, sz = (T)a.sz, cy = (T)a.cy, sy = (T)a.sy;
+    const cx<T> ep = {cz, sz}, en = {cz, -sz};
+    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < a.n_pairs; p += gstride) {
         const uint64_t i0 = insert_zero(p, a.q), i1 = i0 | (1ULL << a.q);
-        const cx<T> x0 = psi[i0], x1 = psi[i1], l0 = lam[i0], l1 = lam[i1];
-        const double x0r = x0.re, x0i = x0.im, x1r = x1.re, x1i = x1.im;
-        const double l0r = l0.re, l0i = l0.im, l1r = l1.re, l1i = l1.im;
-        if (a.axis == 0) {
+        cx<T> x0 = psi[i0], x1 = psi[i1], l0 = lam[i0], l1 = lam[i1];
+        {
+            // <l|Z x> = conj(l0) x0 - conj(l1) x1; Im(conj(l) x) = lr*xi - li*xr
+            const double x0r = x0.re, x0i = x0.im, x1r = x1.re, x1i = x1.im;
+            const double l0r = l0.re, l0i = l0.im, l1r = l1.re, l1i = l1.im;
+            acc[1] += (l0r * x0i - l0i * x0r) - (l1r * x1i - l1i * x1r);
+            // RZ(theta)^dag = diag(e^{+i theta/2}, e^{-i theta/2})
+            x0 = cmul(x0, ep);
+            x1 = cmul(x1, en);
+            l0 = cmul(l0, ep);
+            l1 = cmul(l1, en);
+        }
+        {
             // <l|Y x> = conj(l0)(-i x1) + conj(l1)(i x0); Im of it
-            // conj(l0)(-i x1): Im = -(l0r*x1r + l0i*x1i)
-            // conj(l1)( i x0): Im =  (l1r*x0r + l1i*x0i)
+            const double x0r = x0.re, x0i = x0.im, x1r = x1.re, x1i = x1.im;
+            const double l0r = l0.re, l0i = l0.im, l1r = l1.re, l1i = l1.im;
             acc[0] += (l1r * x0r + l1i * x0i) - (l0r * x1r + l0i * x1i);
             // RY(theta)^dag = [[c, s], [-s, c]]
-            psi[i0] = {c * x0.re + s * x1.re, c * x0.im + s * x1.im};
-            psi[i1] = {c * x1.re - s * x0.re, c * x1.im - s * x0.im};
-            lam[i0] = {c * l0.re + s * l1.re, c * l0.im + s * l1.im};
-            lam[i1] = {c * l1.re - s * l0.re, c * l1.im - s * l0.im};
-        } else {
-            // <l|Z x> = conj(l0) x0 - conj(l1) x1; Im(conj(l) x) = lr*xi - li*xr
-            acc[0] += (l0r * x0i - l0i * x0r) - (l1r * x1i - l1i * x1r);
-            // RZ(theta)^dag = diag(e^{+i theta/2}, e^{-i theta/2})
-            const cx<T> ep = {c, s}, en = {c, -s};
-            psi[i0] = cmul(x0, ep);
-            psi[i1] = cmul(x1, en);
-            lam[i0] = cmul(l0, ep);
-            lam[i1] = cmul(l1, en);
+            psi[i0] = {cy * x0.re + sy * x1.re, cy * x0.im + sy * x1.im};
+            psi[i1] = {cy * x1.re - sy * x0.re, cy * x1.im - sy * x0.im};
+            lam[i0] = {cy * l0.re + sy * l1.re, cy * l0.im + sy * l1.im};
+            lam[i1] = {cy * l1.re - sy * l0.re, cy * l1.im - sy * l0.im};
         }
     }
-    block_sum<1>(acc, part + blockIdx.x);
+    block_sum<2>(acc, part + blockIdx.x, stride);
 }
 
 // CNOT (self-inverse) on two vectors: swap target 0/1 where the control is 1.

@@ -9,7 +9,7 @@ namespace sqk {
 constexpr int kRedBlocks = 2048;
 
 template <int NV>
-__device__ inline void block_sum(double (&v)[NV], double *__restrict__ out) {
+__device__ inline void block_sum(double (&v)[NV], double *__restrict__ out, uint64_t stride = 1) {
     __shared__ double sm[NV][kBlock / 64];
 #pragma unroll
     for (int i = 0; i < NV; i++) {
@@ -24,7 +24,7 @@ __device__ inline void block_sum(double (&v)[NV], double *__restrict__ out) {
         for (int i = 0; i < NV; i++) {
             double t = 0;
             for (int w = 0; w < kBlock / 64; w++) t += sm[i][w];
-            out[i] = t;
+            out[i * stride] = t;
         }
     }
 }

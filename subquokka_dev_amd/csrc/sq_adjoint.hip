@@ -135,21 +135,24 @@ int gradient(sq_engine *e, int layers, const double *theta, int ckpt_every, doub
                 rc = cnot<T>(e, psi, lam, q, (q + 1) % n);
                 if (rc) return rc;
             }
-        for (int half = 1; half >= 0; half--)
-            for (int q = n - 1; q >= 0; q--) {
-                const size_t p = (size_t)l * 2 * n + half * n + q;
-                AdjRotArgs a{};
-                a.n_pairs = na / 2;
-                a.q = q;
-                a.axis = half == 0 ? 0 : 1;
-                a.c = std::cos(theta[p] / 2);
-                a.s = std::sin(theta[p] / 2);
-                int slot = e->stat_begin(half == 0 ? "adj_ry" : "adj_rz", 4.0 * e->amp_bytes() * (double)na);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_rot<T>), grid, kBlock, 0, e->stream, psi, lam, a, part + p * grid);
-                e->stat_end(slot);
-                rc = e->check_launch("k_adj_rot");
-                if (rc) return rc;
-            }
+        // each qubit's RZ then RY, qubits in reverse order (rotations on
+        // different qubits commute)
+        for (int q = n - 1; q >= 0; q--) {
+            const size_t py = (size_t)l * 2 * n + q, pz = py + n;
+            AdjRot2Args a{};
+            a.n_pairs = na / 2;
+            a.q = q;
+            a.cz = std::cos(theta[pz] / 2);
+            a.sz = std::sin(theta[pz] / 2);
+            a.cy = std::cos(theta[py] / 2);
+            a.sy = std::sin(theta[py] / 2);
+            int slot = e->stat_begin("adj_rot2", 4.0 * e->amp_bytes() * (double)na);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_rot2<T>), grid, kBlock, 0, e->stream, psi, lam, a, part + py * grid,
+                               (uint64_t)n * grid);
+            e->stat_end(slot);
+            rc = e->check_launch("k_adj_rot2");
+            if (rc) return rc;
+        }
     }
     std::vector<double> h(n_params * grid);
     hipError_t he = hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream);
