@@ -47,6 +47,7 @@ int low_fixed(int dtype) { return dtype == SQ_C128 ? 6 : 7; }      // one wave's
 
 struct TilePass {
     bool sweep = false;                 // true: apply gates one by one (VSWAP blocks, tiny states)
+    bool plain = false;                 // true: the block as register passes (the plain sweep's form)
     std::vector<int> gates;             // indices into the lowered list
     std::vector<int> q;                 // tile qubits ascending (tile passes)
     int op_begin = 0, n_ops = 0;        // slice of the op array
@@ -117,7 +118,21 @@ int run_blocks(sq_engine *e, const sq_gate *gates, const std::vector<LoweredGate
         if (cnt == 0) continue;
         const int kind0 = gates[first].kind;
         const bool vswap = kind0 == SQ_GATE_VSWAP_1_1 || kind0 == SQ_GATE_VSWAP_2_2;
-        if (vswap || n < kTileMinK || !kTilePassesEnabled) {
+        // A block of one-qubit dense and diagonal gates runs as register
+        // passes: its gates stay in order, each pass reads and writes HBM once,
+        // and the gates are applied in registers rather than in an LDS tile
+        // (whose per-gate LDS traffic bounds the tile pass).
+        bool mergeable = !vswap && e->sweep_mode == 0;
+        for (size_t i = 0; mergeable && i < cnt; i++) {
+            const LoweredGate &g = low[first + i];
+            mergeable = (g.cls == OpClass::Diag && g.k <= 2) || (g.cls == OpClass::Dense && g.k == 1);
+        }
+        if (mergeable) {
+            TilePass p;
+            p.plain = true;
+            for (size_t i = 0; i < cnt; i++) p.gates.push_back((int)(first + i));
+            passes.push_back(std::move(p));
+        } else if (vswap || n < kTileMinK || !kTilePassesEnabled) {
             TilePass p;
             p.sweep = true;
             for (size_t i = 0; i < cnt; i++) p.gates.push_back((int)(first + i));
@@ -165,7 +180,7 @@ int run_blocks(sq_engine *e, const sq_gate *gates, const std::vector<LoweredGate
     std::vector<TileOp> ops;
     std::vector<cx<T>> coef;
     for (auto &p : passes) {
-        if (p.sweep) continue;
+        if (p.sweep || p.plain) continue;
         p.op_begin = (int)ops.size();
         for (int gi : p.gates) encode_op<T>(low[gi], p.q, ops, coef);
         p.n_ops = (int)ops.size() - p.op_begin;
@@ -198,6 +213,17 @@ int run_blocks(sq_engine *e, const sq_gate *gates, const std::vector<LoweredGate
                 int rc = e->apply_lowered(low[gi]);
                 if (rc) return rc;
             }
+            continue;
+        }
+        if (p.plain) {
+            std::vector<sq_gate> bg;
+            std::vector<LoweredGate> bl;
+            for (int gi : p.gates) {
+                bg.push_back(gates[gi]);
+                bl.push_back(low[gi]);
+            }
+            int rc = e->apply_plain(bg, bl);
+            if (rc) return rc;
             continue;
         }
         TileArgs<T> a{};

@@ -39,6 +39,7 @@ int main(int argc, char **argv) {
     cfg.total_qbit = n; cfg.chunk_qbit = ooc ? 0 : chunk; cfg.dtype = dtype;
     if (ooc) cfg.resident_qbit = chunk;
     if (const char *kh = std::getenv("EMUL_PASS_QBITS")) cfg.pass_qbits = std::atoi(kh);
+    if (const char *sm = std::getenv("EMUL_SWEEP_MODE")) cfg.sweep_mode = std::atoi(sm);
     sq_engine *e = nullptr;
     const char *seed_env = std::getenv("EMUL_INIT_SEED");
     // out-of-core host tier: EMUL_HOST_TIER = pageable | files (files in EMUL_FILES_DIR)

@@ -36,7 +36,7 @@ def emul():
     return EMUL
 
 
-def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None, tier=None, pass_qbits=None):
+def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None, tier=None, pass_qbits=None, sweep_mode=None):
     f = tmp_path / "c.txt"
     f.write_text(text)
     out = tmp_path / f"state_{tier or 'default'}.bin"
@@ -45,6 +45,8 @@ def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None, tier=None, 
         env["EMUL_INIT_SEED"] = str(seed)
     if pass_qbits is not None:
         env["EMUL_PASS_QBITS"] = str(pass_qbits)
+    if sweep_mode is not None:
+        env["EMUL_SWEEP_MODE"] = str(sweep_mode)
     if tier is not None:
         env["EMUL_HOST_TIER"] = tier
         env["EMUL_FILES_DIR"] = str(tmp_path)
@@ -56,17 +58,19 @@ def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None, tier=None, 
     return raw[0::2] + 1j * raw[1::2]
 
 
+@pytest.mark.parametrize("sweep_mode", [0, 1])   # 1: every gate block as an LDS tile
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-def test_sub_fixture_qft21_tiles(emul, tmp_path, dtype):
+def test_sub_fixture_qft21_tiles(emul, tmp_path, dtype, sweep_mode):
     text = open(os.path.join(GOLDEN, "ref_circuits", "sub_qft21.txt")).read()
-    run_case(emul, tmp_path, "sub", 21, 12, dtype, text)
+    run_case(emul, tmp_path, "sub", 21, 12, dtype, text, sweep_mode=sweep_mode)
 
 
+@pytest.mark.parametrize("sweep_mode", [0, 1])
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-def test_planned_blocks_tiles(emul, tmp_path, dtype):
+def test_planned_blocks_tiles(emul, tmp_path, dtype, sweep_mode):
     n = 14
     text = sq.plan(random_circuit(n, 200, seed=3), n, 1, 5)
-    run_case(emul, tmp_path, "sub", n, 12, dtype, text)
+    run_case(emul, tmp_path, "sub", n, 12, dtype, text, sweep_mode=sweep_mode)
 
 
 def test_wide_block_split(emul, tmp_path):

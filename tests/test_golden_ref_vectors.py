@@ -33,11 +33,13 @@ def test_oracle_reproduces_reference_vectors(oracle, name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("sweep_mode", [0, 1])
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("name", CASES)
-def test_engine_reproduces_reference_vectors(name, dtype):
+def test_engine_reproduces_reference_vectors(name, dtype, sweep_mode):
+    # sweep_mode 1: one launch per gate (plain lists) and LDS tiles for every gate block
     n, chunk, sub, s_in, s_out, text = load(name)
-    eng = sq.Engine(n, dtype=dtype, chunk=chunk)
+    eng = sq.Engine(n, dtype=dtype, chunk=chunk, sweep_mode=sweep_mode)
     try:
         eng.set_state(s_in.astype(np.complex128 if dtype == "c128" else np.complex64))
         gates, blocks, ng, nb = sq.parse_circuit(text, n, is_subcircuit=sub)

@@ -26,8 +26,10 @@ def say(msg):
     sys.stdout.flush()
 
 
-def run_gpu(n, text, state, dtype="c128", sub=False, chunk=0):
-    eng = sq.Engine(n, dtype=dtype, chunk=chunk)
+def run_gpu(n, text, state, dtype="c128", sub=False, chunk=0, sweep_mode=0):
+    # blocked sweeps: sweep_mode 0 runs blocks of one-qubit dense and diagonal
+    # gates as register passes, sweep_mode 1 runs every gate block as an LDS tile
+    eng = sq.Engine(n, dtype=dtype, chunk=chunk, sweep_mode=sweep_mode)
     try:
         eng.set_state(state.astype(np.complex128 if dtype == "c128" else np.complex64))
         gates, blocks, ng, nb = sq.parse_circuit(text, n, is_subcircuit=sub)
@@ -164,14 +166,15 @@ def test_config1_qft20_seeded_state(oracle, dtype):
     check(run_gpu(n, text, s0, dtype), want, dtype, "qft20")
 
 
+@pytest.mark.parametrize("sweep_mode", [0, 1])
 @pytest.mark.parametrize("circ", ["qft", "qaoa", "h", "rx", "rzz"])
-def test_reference_sub_fixtures_blocked(oracle, circ):
+def test_reference_sub_fixtures_blocked(oracle, circ, sweep_mode):
     n = 21
     text = open(os.path.join(GOLDEN, "ref_circuits", f"sub_{circ}21.txt")).read()
     s0 = random_state(n, 21)
     say(f"sub fixture {circ}21")
     want = oracle.run(s0, n, text, chunk=12, sub=True)
-    check(run_gpu(n, text, s0, "c128", sub=True, chunk=12), want, "c128", circ)
+    check(run_gpu(n, text, s0, "c128", sub=True, chunk=12, sweep_mode=sweep_mode), want, "c128", circ)
 
 
 def test_reductions(oracle):
@@ -240,32 +243,36 @@ def test_large_state_round_trip_property():
         eng.close()
 
 
+@pytest.mark.parametrize("sweep_mode", [0, 1])
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("n,k", [(14, 5), (18, 4), (20, 6)])
-def test_planned_blocks_in_lds_tiles(oracle, dtype, n, k):
+def test_planned_blocks_in_lds_tiles(oracle, dtype, n, k, sweep_mode):
     # planner blocks (<= k qubits each) run as LDS tile passes; same state as plain
     text = random_circuit(n, 400, seed=n + k)
     plan = sq.plan(text, n, 1, k)
     s0 = random_state(n, n)
     say(f"tiles n={n} k={k} {dtype}")
-    check(run_gpu(n, plan, s0, dtype, sub=True), oracle.run(s0, n, text), dtype, f"tiles n={n}")
+    check(run_gpu(n, plan, s0, dtype, sub=True, sweep_mode=sweep_mode), oracle.run(s0, n, text), dtype, f"tiles n={n}")
 
 
+@pytest.mark.parametrize("sweep_mode", [0, 1])
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-def test_wide_block_is_split_into_tiles(oracle, dtype):
+def test_wide_block_is_split_into_tiles(oracle, dtype, sweep_mode):
     # one block touching all 16 qubits: wider than any tile, split by the planner
     n = 16
     text = random_circuit(n, 300, seed=5)
     sub = f"{len(text.strip().splitlines())}\n" + text
     s0 = random_state(n, 6)
-    check(run_gpu(n, sub, s0, dtype, sub=True), oracle.run(s0, n, text), dtype, "wide block")
+    check(run_gpu(n, sub, s0, dtype, sub=True, sweep_mode=sweep_mode), oracle.run(s0, n, text), dtype, "wide block")
 
 
+@pytest.mark.parametrize("sweep_mode", [0, 1])
 @pytest.mark.parametrize("n,chunk", [(16, 10), (20, 12), (24, 12)])
-def test_finder_plan_vswap_blocks(oracle, n, chunk):
+def test_finder_plan_vswap_blocks(oracle, n, chunk, sweep_mode):
     # planner in the finder's format (blocks + VSWAP passes, order restored)
     text = random_circuit(n, 250, seed=chunk)
     plan = sq.plan(text, n, 3, chunk)
     s0 = random_state(n, 2)
     say(f"finder plan n={n} chunk={chunk}")
-    check(run_gpu(n, plan, s0, "c128", sub=True, chunk=chunk), oracle.run(s0, n, text), "c128", "finder plan")
+    check(run_gpu(n, plan, s0, "c128", sub=True, chunk=chunk, sweep_mode=sweep_mode), oracle.run(s0, n, text), "c128",
+          "finder plan")
