@@ -1,17 +1,29 @@
 // sq_blocks.hip -- subcircuit (blocked) sweep: MEM_Runner::run(subcircuits)
 // (MEM_Runner.cpp:136-185).
 //
-// A subcircuit file alternates VSWAP blocks (qubit-swap passes, applied as
-// permutation sweeps) and gate blocks.  The reference applies a gate block
-// tile by tile (MEM_Runner.cpp:173-182); here a gate block becomes one LDS
-// tile pass (kernels/tile.hpp) over a tile qubit set Q that contains the
-// block's targets and the lowest index bits.  A block too wide for one tile
-// is split by the planner into several tile passes; the result is the same
-// because the gates are applied in an order that respects every per-qubit
-// dependency.
+// A subcircuit file alternates VSWAP blocks (qubit-swap passes) and gate
+// blocks.  The reference applies a gate block tile by tile
+// (MEM_Runner.cpp:173-182) and physically moves the data at every VSWAP so
+// that the next block's targets fall in its cache-sized chunk.
+//
+// With the whole state in HBM there is no chunk to fill.  The default sweep
+// (sweep_mode 0) therefore leaves the data where it is: it records the
+// permutation the VSWAPs make (sigma[p] = the index bit that holds the
+// reference's bit p), applies every gate on its mapped targets as the plain
+// sweep does (register passes), and moves the data once at the end, with
+// qubit swaps, so the final layout is exactly the reference's.  A plan whose
+// swaps restore the qubit order moves no data at all.
+//
+// sweep_mode 1 keeps the reference's shape: VSWAP blocks are permutation
+// sweeps and each gate block is one LDS tile pass (kernels/tile.hpp) over a
+// tile qubit set Q that contains the block's targets and the lowest index
+// bits.  A block too wide for one tile is split by the planner into several
+// tile passes; the result is the same because the gates are applied in an
+// order that respects every per-qubit dependency.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <complex>
 #include <string>
 #include <vector>
 
@@ -100,6 +112,95 @@ void encode_op(const LoweredGate &g, const std::vector<int> &q, std::vector<Tile
         push_coef(g.k == 1 ? 2 : 4);
     }
     ops.push_back(o);
+}
+
+// The gate as it acts after the data moves the VSWAPs so far would have made:
+// targets mapped through sigma.  A two-qubit gate keeps its internal basis
+// (local bit j <-> j-th smallest target), so when the mapping reverses the
+// order of its targets, its matrix or diagonal is permuted to match.
+void relabel(LoweredGate &g, const std::vector<int> &sigma) {
+    if (g.cls == OpClass::Perm) {
+        for (int i = 0; i < 2 * g.n_pairs; i++) g.t[i] = sigma[g.t[i]];
+        return;
+    }
+    if (g.k == 1) {
+        g.t[0] = sigma[g.t[0]];
+        return;
+    }
+    const int a = sigma[g.t[0]], b = sigma[g.t[1]];
+    if (a < b) {
+        g.t[0] = a;
+        g.t[1] = b;
+        return;
+    }
+    g.t[0] = b;
+    g.t[1] = a;
+    auto sw = [](int l) { return ((l & 1) << 1) | ((l >> 1) & 1); };
+    if (g.cls == OpClass::Dense) {
+        std::complex<double> m[16];
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++) m[sw(r) * 4 + sw(c)] = g.m[r * 4 + c];
+        std::copy(m, m + 16, g.m);
+    } else {
+        std::complex<double> d[4];
+        uint32_t act = 0;
+        for (int l = 0; l < 4; l++) {
+            d[sw(l)] = g.m[l];
+            act |= ((g.diag_active >> l) & 1u) << sw(l);
+        }
+        std::copy(d, d + 4, g.m);
+        g.diag_active = act;
+    }
+}
+
+int run_blocks_mapped(sq_engine *e, const sq_gate *gates, const std::vector<LoweredGate> &low,
+                      const uint32_t *block_sizes, uint64_t n_blocks) {
+    const int n = e->n;
+    std::vector<int> sigma(n);
+    for (int p = 0; p < n; p++) sigma[p] = p;
+    std::vector<sq_gate> flat;
+    std::vector<LoweredGate> flat_low;
+    size_t first = 0;
+    for (uint64_t b = 0; b < n_blocks; b++) {
+        for (size_t i = first; i < first + block_sizes[b]; i++) {
+            const sq_gate &g = gates[i];
+            if (g.kind == SQ_GATE_VSWAP_1_1 || g.kind == SQ_GATE_VSWAP_2_2) {
+                // the reference's bit a now holds what its bit b held, and b what a held
+                for (int j = 0; j < low[i].n_pairs; j++) std::swap(sigma[low[i].t[2 * j]], sigma[low[i].t[2 * j + 1]]);
+                continue;
+            }
+            LoweredGate lg = low[i];
+            relabel(lg, sigma);
+            sq_gate mg = g;
+            for (int j = 0; j < g.nq; j++) mg.q[j] = sigma[g.q[j]];
+            flat.push_back(mg);
+            flat_low.push_back(lg);
+        }
+        first += block_sizes[b];
+    }
+    int rc = e->apply_plain(flat, flat_low);
+    if (rc) return rc;
+    // move the data to the reference's layout: swap index bits until sigma is
+    // the identity (at most n - 1 swaps)
+    for (int p = 0; p < n; p++) {
+        while (sigma[p] != p) {
+            const int x = p, y = sigma[p];
+            sq_gate sw{};
+            sw.kind = SQ_GATE_SWAP;
+            sw.nq = 2;
+            sw.q[0] = std::min(x, y);
+            sw.q[1] = std::max(x, y);
+            LoweredGate lg;
+            rc = e->lower(sw, lg);
+            if (!rc) rc = e->apply_lowered(lg);
+            if (rc) return rc;
+            for (int r = 0; r < n; r++) {
+                if (sigma[r] == x) sigma[r] = y;
+                else if (sigma[r] == y) sigma[r] = x;
+            }
+        }
+    }
+    return SQ_OK;
 }
 
 template <typename T>
@@ -259,6 +360,7 @@ extern "C" int sq_apply_blocks(sq_engine *e, const sq_gate *gates, const uint32_
     }
     if (e->out_of_core())   // same map as the gates in order
         return e->stream_apply(std::vector<sq_gate>(gates, gates + total), low);
+    if (e->sweep_mode == 0) return run_blocks_mapped(e, gates, low, block_sizes, n_blocks);
     return e->dtype == SQ_C128 ? run_blocks<double>(e, gates, low, block_sizes, n_blocks)
                                : run_blocks<float>(e, gates, low, block_sizes, n_blocks);
 }
