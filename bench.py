@@ -50,11 +50,12 @@ def parse():
     ap.add_argument("--fused-dtypes", default="c64,c128")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--workload", default="random30",
-                    choices=["random30", "fused32", "stream", "qft20", "traffic"],
+                    choices=["random30", "fused32", "stream", "qft20", "blocks30", "traffic"],
                     help="random30: the headline line (default).  fused32: 32 qubits, 24 layers, "
                          "fused complex64 vs plain complex128.  stream: out-of-core runner "
                          "(state in pinned host memory) vs the same circuit in HBM.  qft20: "
-                         "20-qubit QFT on a seeded state.  traffic: a short fixed kernel "
+                         "20-qubit QFT on a seeded state.  blocks30: the 30-qubit circuit as a "
+                         "chunk-12 subcircuit file.  traffic: a short fixed kernel "
                          "sequence for rocprofv3 --pmc HBM-byte counts")
     ap.add_argument("--stream-dtype", default="c128")
     ap.add_argument("--resident-qbit", type=int, default=30)
@@ -293,6 +294,60 @@ def workload_qft20(args):
                         "state": "complex128 in HBM (16 MiB)"})
 
 
+def workload_blocks30(args):
+    """The 30-qubit random-layer circuit as a subcircuit file (finder format,
+    chunk_qbit 12: gate blocks and VSWAP blocks, no order restore) through
+    sq_apply_blocks, complex128 in HBM.  The default sweep relabels the VSWAPs
+    on the host and applies one set of qubit swaps at the end; the
+    reference-shaped sweep (physical VSWAP passes and one LDS tile pass per
+    gate block) is timed beside it, and the two final states compared."""
+    import subquokka_dev_amd as sq
+    n, layers, chunk = 30, 20, 12
+    text = sq.gen_circuit("random", n, layers, seed=SEED)
+    plan = sq.plan(text, n, 0, chunk)
+    gates, blocks, ng, nb = sq.parse_circuit(plan, n, is_subcircuit=True)
+    n_vswap = sum(1 for l in plan.splitlines() if l.startswith("VSWAP"))
+
+    def timed(sweep_mode, steps, warmup):
+        eng = sq.Engine(n, dtype="c128", chunk=chunk, sweep_mode=sweep_mode)
+        for _ in range(warmup):
+            eng.init_zero()
+            eng.apply_blocks(gates, blocks, nb)
+        eng.sync()
+        times, st = [], None
+        for _ in range(steps):
+            eng.init_zero()
+            eng.sync()
+            eng.stats_reset()
+            eng.stats_enable(True)
+            t0 = time.perf_counter()
+            eng.apply_blocks(gates, blocks, nb)
+            eng.sync()
+            times.append(time.perf_counter() - t0)
+            st = eng.stats()
+        return eng, sum(times) / len(times), st
+
+    def kstats(st):
+        return {k: {"launches": v["launches"], "avg_ms": round(v["total_ms"] / v["launches"], 3),
+                    "gbs": round(v["alg_bytes"] / (v["total_ms"] * 1e-3) / 1e9, 1)} for k, v in st.items()}
+
+    e0, t0, st0 = timed(0, args.steps, args.warmup)
+    e1, t1, st1 = timed(1, 1, 1)
+    diff = e0.max_abs_diff(e1)
+    e0.close()
+    e1.close()
+    out = line("30q random circuit as a chunk-12 subcircuit file: gates/s", (ng - n_vswap) / t0, "gates/s",
+               args.steps, args.warmup, t0 * 1e3, "f64",
+               {"workload": f"random_layers_{n}q_{layers}l_seed{SEED}_blocks_chunk{chunk}_c128", "n_qubits": n,
+                "gates": ng - n_vswap, "vswap_blocks": n_vswap, "blocks": nb, "state": "complex128 in HBM",
+                "mode": "MEM_Runner::run(vector<vector<Gate*>>&) form: VSWAPs relabelled, register passes"})
+    out["kernels"] = kstats(st0)
+    out["reference_shaped_sweep"] = {"ms_per_step": round(t1 * 1e3, 1), "kernels": kstats(st1),
+                                     "what": "physical VSWAP sweeps + one LDS tile pass per gate block",
+                                     "max_abs_diff_vs_default": diff}
+    return out
+
+
 def workload_traffic(args):
     """A short fixed kernel sequence whose HBM bytes per dispatch rocprofv3
     --pmc reads (FETCH_SIZE, WRITE_SIZE): a 4 GiB device copy of known size
@@ -353,7 +408,7 @@ def main():
     import numpy as np
     if args.workload != "random30":
         fn = {"fused32": workload_fused32, "stream": workload_stream, "qft20": workload_qft20,
-              "traffic": workload_traffic}[args.workload]
+              "blocks30": workload_blocks30, "traffic": workload_traffic}[args.workload]
         if args.workload == "stream" and args.qubits == N_QUBITS:
             args.qubits = 33
         print(json.dumps(fn(args)), flush=True)
