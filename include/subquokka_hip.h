@@ -99,9 +99,11 @@ typedef struct sq_config {
     int32_t resident_qbit;    /* 0: whole state in HBM (MEM runner).  >0: out-of-core
                                  runner, state in pinned host memory, 2^resident_qbit
                                  amplitudes per streamed chunk (DIO_Runner role) */
-    int32_t sweep_mode;       /* plain sweep: 0 = register passes (a run of one-qubit
-                                 and diagonal gates per read/write of the state),
-                                 1 = one launch per gate */
+    int32_t sweep_mode;       /* 0 = register passes (a run of one-qubit and diagonal
+                                 gates per read/write of the state); subcircuits with
+                                 VSWAPs relabelled and one set of qubit swaps at the end.
+                                 1 = one launch per gate; subcircuits as VSWAP sweeps
+                                 and one LDS tile pass per gate block */
     int32_t pass_qbits;       /* register qubits per pass, 1..5 (0: 5 for SQ_C128, 4 for SQ_C64) */
     int32_t host_tier;        /* out-of-core runner: SQ_HOST_AUTO or SQ_HOST_PAGEABLE */
 } sq_config;
@@ -140,7 +142,8 @@ const char *sq_last_error(const sq_engine *e);
 int sq_apply_gates(sq_engine *e, const sq_gate *gates, uint64_t n);
 /* Subcircuit sweep: MEM_Runner::run(vector<vector<Gate*>>&) (MEM_Runner.cpp:136-185).
  * block_sizes[i] gates form block i; blocks whose first gate is VSWAP_* are
- * qubit-swap passes, the others are applied tile by tile in LDS. */
+ * qubit-swap passes, the others gate blocks.  The final state, in the layout the
+ * VSWAPs leave, is the reference's; how it gets there follows sweep_mode. */
 int sq_apply_blocks(sq_engine *e, const sq_gate *gates, const uint32_t *block_sizes,
                     uint64_t n_blocks);
 /* Dense fused blocks on MFMA (no reference counterpart, see sq_fused).
