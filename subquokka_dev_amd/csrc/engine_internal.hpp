@@ -17,6 +17,21 @@ struct PendingStat {
     hipEvent_t start, stop;
 };
 
+// A plain gate list as register passes, planned and its op program uploaded
+// once; it runs on any state buffer of the size (n qubits) it was planned for.
+struct PlainLaunch {
+    bool merged = false;
+    int gate = -1;                 // single-gate step: index into the lowered list
+    int op_begin = 0, n_ops = 0;   // merged step: slice of the uploaded ops
+    double flops_per_amp = 0;      // algorithmic real flops per amplitude
+    int hpos[5] = {};              // register qubits of the pass
+};
+struct PlainProgram {
+    int n = 0, kh = 0;
+    bool per_gate = false;         // one launch per gate (sweep_mode 1, small states)
+    std::vector<PlainLaunch> launches;
+};
+
 struct sq_engine {
     int n = 0;                  // total_qbit
     int chunk = 12;             // chunk_qbit (tile width of the subcircuit sweep)
@@ -98,6 +113,8 @@ struct sq_engine {
     int lower(const sq_gate &g, sq::LoweredGate &lg);
     int apply_lowered(const sq::LoweredGate &g);
     int apply_plain(const std::vector<sq_gate> &gates, const std::vector<sq::LoweredGate> &low);
+    int build_plain(const std::vector<sq_gate> &gates, const std::vector<sq::LoweredGate> &low, PlainProgram &prog);
+    int run_plain_program(const PlainProgram &prog, const std::vector<sq::LoweredGate> &low);
     int ensure_partials();
     int norm_zz(double *v);
     int compare(sq_engine *o, double *v);

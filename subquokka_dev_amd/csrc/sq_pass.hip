@@ -27,13 +27,7 @@ using namespace sqk;
 
 namespace {
 
-struct PassLaunch {
-    bool merged = false;
-    int gate = -1;                 // single-gate step
-    int op_begin = 0, n_ops = 0;   // merged step
-    double flops_per_amp = 0;      // algorithmic real flops per amplitude
-    int hpos[kPassMaxKH] = {};
-};
+static_assert(kPassMaxKH <= 5, "PlainLaunch holds five register qubits");
 
 template <typename T>
 void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vector<PassOp<T>> &ops) {
@@ -126,16 +120,17 @@ int launch_pass(sq_engine *e, const PassArgs<T> &a) {
 }
 
 template <typename T>
-int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low) {
+int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low,
+                  PlainProgram &prog) {
     constexpr int VB = Traits<T>::VB;
     constexpr int S = VB + 6;
     const int n = e->n;
     const int kh = e->pass_qbits;
+    prog = PlainProgram{};
+    prog.n = n;
+    prog.kh = kh;
     if (e->sweep_mode == 1 || n < S + kh) {
-        for (const auto &g : low) {
-            int rc = e->apply_lowered(g);
-            if (rc) return rc;
-        }
+        prog.per_gate = true;
         return SQ_OK;
     }
     const size_t ng = gates.size();
@@ -147,16 +142,15 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
     }
     const auto steps = sq::plan_register_passes(gates, mergeable, anywhere, n, S, kh);
 
-    std::vector<PassLaunch> launches;
     std::vector<PassOp<T>> ops;
     for (const auto &st : steps) {
         double frac = 0;
         for (int gi : st.gates) frac += low[gi].pass_fraction;
         if (!st.merged || frac <= 1.0) {
             for (int gi : st.gates) {
-                PassLaunch l;
+                PlainLaunch l;
                 l.gate = gi;
-                launches.push_back(l);
+                prog.launches.push_back(l);
             }
             continue;
         }
@@ -165,7 +159,7 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
         for (int q = S; q < n && (int)regs.size() < kh; q++)
             if (std::find(regs.begin(), regs.end(), q) == regs.end()) regs.push_back(q);
         std::sort(regs.begin(), regs.end());
-        PassLaunch l;
+        PlainLaunch l;
         l.merged = true;
         l.op_begin = (int)ops.size();
         for (int gi : st.gates) {
@@ -175,7 +169,7 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
         }
         l.n_ops = (int)ops.size() - l.op_begin;
         for (int j = 0; j < kh; j++) l.hpos[j] = regs[j];
-        launches.push_back(l);
+        prog.launches.push_back(l);
     }
 
     // upload the program once; earlier launches may still read the old one
@@ -194,12 +188,29 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
         hipError_t he = hipMemcpy(d, ops.data(), ops.size() * sizeof(PassOp<T>), hipMemcpyHostToDevice);
         if (he != hipSuccess) return e->hip_fail(he, "hipMemcpy(pass program)");
     }
-    const PassOp<T> *dops = reinterpret_cast<const PassOp<T> *>(e->d_passprog);
+    return SQ_OK;
+}
 
+// Runs a program on the engine's current state buffer (e->n must be the
+// program's n); the program's ops are the last ones uploaded.
+template <typename T>
+int run_plain_t(sq_engine *e, const PlainProgram &prog, const std::vector<LoweredGate> &low) {
+    constexpr int VB = Traits<T>::VB;
+    constexpr int S = VB + 6;
+    const int n = e->n, kh = prog.kh;
+    if (n != prog.n) return e->fail(SQ_ERR_ARG, "internal: pass program planned for another size");
+    if (prog.per_gate) {
+        for (const auto &g : low) {
+            int rc = e->apply_lowered(g);
+            if (rc) return rc;
+        }
+        return SQ_OK;
+    }
+    const PassOp<T> *dops = reinterpret_cast<const PassOp<T> *>(e->d_passprog);
     char nm[48];
     std::snprintf(nm, sizeof nm, "pass%d_%s", kh, e->dtype == SQ_C128 ? "c128" : "c64");
     const double bytes = 2.0 * e->amp_bytes() * std::ldexp(1.0, n);
-    for (const auto &l : launches) {
+    for (const auto &l : prog.launches) {
         if (!l.merged) {
             int rc = e->apply_lowered(low[l.gate]);
             if (rc) return rc;
@@ -229,6 +240,16 @@ int run_plain(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector
 
 }  // namespace
 
+int sq_engine::build_plain(const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low, PlainProgram &prog) {
+    return dtype == SQ_C128 ? build_plain_t<double>(this, gates, low, prog) : build_plain_t<float>(this, gates, low, prog);
+}
+
+int sq_engine::run_plain_program(const PlainProgram &prog, const std::vector<LoweredGate> &low) {
+    return dtype == SQ_C128 ? run_plain_t<double>(this, prog, low) : run_plain_t<float>(this, prog, low);
+}
+
 int sq_engine::apply_plain(const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low) {
-    return dtype == SQ_C128 ? run_plain<double>(this, gates, low) : run_plain<float>(this, gates, low);
+    PlainProgram prog;
+    int rc = build_plain(gates, low, prog);
+    return rc ? rc : run_plain_program(prog, low);
 }

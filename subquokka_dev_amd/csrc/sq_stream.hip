@@ -416,7 +416,20 @@ int sq_engine::stream_apply(const std::vector<sq_gate> &gates, const std::vector
         for (int q = 0; q < n; q++) if (in[q]) { P.push_back(q); if (q >= W - h) H.push_back(q); }
         if ((int)P.size() != W || (int)H.size() != h) { rc = fail(SQ_ERR_ARG, "internal: bad pass qubit set"); break; }
         std::vector<LoweredGate> pg;
-        for (int gi : pass.gates) pg.push_back(remap(low[gi], P));
+        std::vector<sq_gate> pgates;
+        for (int gi : pass.gates) {
+            pg.push_back(remap(low[gi], P));
+            sq_gate g = gates[gi];
+            for (int j = 0; j < g.nq; j++) g.q[j] = (int)(std::lower_bound(P.begin(), P.end(), g.q[j]) - P.begin());
+            pgates.push_back(g);
+        }
+        // the pass's gates as register passes over a W-qubit group: planned and
+        // uploaded once, run on every group
+        PlainProgram prog;
+        n = W;
+        rc = build_plain(pgates, pg, prog);
+        n = saved_n;
+        if (rc) break;
         std::vector<uint64_t> run_off(1ULL << h), run_off_bytes(1ULL << h);   // host offset of run r within a group
         for (uint64_t r = 0; r < run_off.size(); r++) {
             uint64_t o = 0;
@@ -459,10 +472,7 @@ int sq_engine::stream_apply(const std::vector<sq_gate> &gates, const std::vector
             hipStreamWaitEvent(stream, ev_loaded[b], 0);
             d_state = d_group[b];
             n = W;
-            for (const auto &lg : pg) {
-                rc = apply_lowered(lg);
-                if (rc) break;
-            }
+            rc = run_plain_program(prog, pg);
             n = saved_n;
             d_state = saved_state;
             if (rc) break;
