@@ -267,6 +267,24 @@ def test_wide_block_is_split_into_tiles(oracle, dtype, sweep_mode):
 
 
 @pytest.mark.parametrize("sweep_mode", [0, 1])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n,chunk", [(16, 10), (20, 12)])
+def test_finder_plan_without_restore(oracle, n, chunk, dtype, sweep_mode):
+    # a plan that ends with its qubits permuted (no restore swaps): the final
+    # layout is the one the VSWAPs leave.  sweep_mode 0 relabels the gates and
+    # swaps qubits once at the end; sweep_mode 1 moves the data at every VSWAP.
+    # The oracle applies the file VSWAP by VSWAP.  Two-qubit gates (U2, SWAP,
+    # CP, RZZ) meet relabellings that reverse their targets.
+    text = random_circuit(n, 300, seed=chunk + 7)
+    plan = sq.plan(text, n, 0, chunk)
+    assert "VSWAP" in plan
+    s0 = random_state(n, 4)
+    say(f"plan without restore n={n} chunk={chunk} {dtype} sweep_mode={sweep_mode}")
+    want = oracle.run(s0, n, plan, chunk=chunk, sub=True)
+    check(run_gpu(n, plan, s0, dtype, sub=True, chunk=chunk, sweep_mode=sweep_mode), want, dtype, "plan without restore")
+
+
+@pytest.mark.parametrize("sweep_mode", [0, 1])
 @pytest.mark.parametrize("n,chunk", [(16, 10), (20, 12), (24, 12)])
 def test_finder_plan_vswap_blocks(oracle, n, chunk, sweep_mode):
     # planner in the finder's format (blocks + VSWAP passes, order restored)
