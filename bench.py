@@ -555,6 +555,19 @@ def main():
                             "ms_per_step": round(sum(t64) / args.steps * 1e3, 2),
                             "infidelity_vs_c128": infid, "max_abs_err_vs_c128": eng128.max_abs_diff(eng64)}
         eng64.close()
+    # measurement sampler on the final 30-qubit state: one pass over the state
+    # (block sums), then one wave per shot reading its 16 KiB block
+    shots = 1 << 20
+    eng128.sample(1000, 7)
+    eng128.sync()
+    ts = []
+    for _ in range(3):
+        t0s = time.perf_counter()
+        eng128.sample(shots, 7)
+        ts.append(time.perf_counter() - t0s)
+    out["sampler"] = {"shots": shots, "ms": round(min(ts) * 1e3, 2), "shots_per_s": round(shots / min(ts), 1),
+                      "state": "the circuit's final complex128 state, 2^30 amplitudes",
+                      "what": "sq_sample: exact seeded shots (FP64 sums), shots copied to the host"}
     eng128.close()
 
     if not args.no_adjoint:

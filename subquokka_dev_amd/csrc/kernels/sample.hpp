@@ -56,6 +56,41 @@ struct SampleShot {
     double r;          // residual inside the block
 };
 
+// One thread per shot: the shot's uniform and its block.  Shot k of a seed
+// uses SplitMix64 output k, i.e. the mix of seed + (k + 1) * gamma, so every
+// shot is drawn independently of the others and equals the sequential
+// generator's.  u = uniform * total (one rounding); the block is the first
+// whose inclusive prefix exceeds u (the comparisons of std::upper_bound), or
+// the last block of nonzero weight when none does; r = u - exclusive prefix.
+struct SampleLocateArgs {
+    uint64_t seed, k0, n_shots, n_blocks, last_nz;
+    double total;
+    const double *incl, *excl;   // inclusive / exclusive block prefixes (sequential FP64 sums)
+    SampleShot *shots;
+};
+
+__global__ void __launch_bounds__(kBlock) k_sample_locate(const SampleLocateArgs a) {
+    const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= a.n_shots) return;
+    uint64_t z = a.seed + (a.k0 + k + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    const double u = __dmul_rn((double)(z >> 11) * 0x1.0p-53, a.total);
+    uint64_t first = 0, count = a.n_blocks;
+    while (count > 0) {
+        const uint64_t step = count / 2, it = first + step;
+        if (!(u < a.incl[it])) {
+            first = it + 1;
+            count -= step + 1;
+        } else {
+            count = step;
+        }
+    }
+    const uint64_t b = first >= a.n_blocks ? a.last_nz : first;
+    a.shots[k] = SampleShot{b, __dsub_rn(u, a.excl[b])};
+}
+
 // One wave per shot: segment sums, sequential prefix by lane 0, then a
 // sequential scan of the chosen segment.
 template <typename T>

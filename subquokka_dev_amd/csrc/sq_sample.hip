@@ -8,7 +8,6 @@
 
 #include "subquokka_hip.h"
 #include "engine_internal.hpp"
-#include "host/rng.hpp"
 #include "kernels/sample.hpp"
 
 using namespace sqk;
@@ -25,7 +24,7 @@ int run_sample(sq_engine *e, uint64_t seed, uint64_t shots, uint64_t *out) {
     const uint64_t batch = std::min(shots, kShotBatch);
     const size_t blk_b = ((n_blocks * sizeof(double) + 255) / 256) * 256;
     const size_t shot_b = ((batch * sizeof(SampleShot) + 255) / 256) * 256;
-    const size_t need = blk_b + shot_b + batch * sizeof(uint64_t);
+    const size_t need = 3 * blk_b + shot_b + batch * sizeof(uint64_t);
     if (hipStreamSynchronize(e->stream) != hipSuccess) return e->fail(SQ_ERR_HIP, "stream sync");
     if (need > e->sample_cap) {
         if (e->d_sample) hipFree(e->d_sample);
@@ -37,8 +36,10 @@ int run_sample(sq_engine *e, uint64_t seed, uint64_t shots, uint64_t *out) {
     }
     char *base = static_cast<char *>(e->d_sample);
     double *d_bsum = reinterpret_cast<double *>(base);
-    SampleShot *d_shots = reinterpret_cast<SampleShot *>(base + blk_b);
-    uint64_t *d_out = reinterpret_cast<uint64_t *>(base + blk_b + shot_b);
+    double *d_incl = reinterpret_cast<double *>(base + blk_b);
+    double *d_excl = reinterpret_cast<double *>(base + 2 * blk_b);
+    SampleShot *d_shots = reinterpret_cast<SampleShot *>(base + 3 * blk_b);
+    uint64_t *d_out = reinterpret_cast<uint64_t *>(base + 3 * blk_b + shot_b);
     const cx<T> *s = static_cast<const cx<T> *>(e->d_state);
 
     const unsigned g1 = (unsigned)((n_blocks + kBlock / 64 - 1) / (kBlock / 64));
@@ -61,19 +62,25 @@ int run_sample(sq_engine *e, uint64_t seed, uint64_t shots, uint64_t *out) {
     const double total = acc;
     if (!(total > 0.0)) return e->fail(SQ_ERR_ARG, "the state has zero norm");
 
-    sq::SplitMix64 rng{seed};
-    std::vector<SampleShot> hs(batch);
+    he = hipMemcpy(d_incl, I.data(), n_blocks * sizeof(double), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(d_excl, C.data(), n_blocks * sizeof(double), hipMemcpyHostToDevice);
+    if (he != hipSuccess) return e->hip_fail(he, "block prefixes");
     for (uint64_t k0 = 0; k0 < shots; k0 += batch) {
         const uint64_t cnt = std::min(batch, shots - k0);
-        for (uint64_t k = 0; k < cnt; k++) {
-            const double u = rng.uniform() * total;
-            uint64_t b = (uint64_t)(std::upper_bound(I.begin(), I.end(), u) - I.begin());
-            if (b >= n_blocks) b = last_nz;
-            hs[k].block = b;
-            hs[k].r = u - C[b];
-        }
-        he = hipMemcpy(d_shots, hs.data(), cnt * sizeof(SampleShot), hipMemcpyHostToDevice);
-        if (he != hipSuccess) return e->hip_fail(he, "shot upload");
+        SampleLocateArgs la{};
+        la.seed = seed;
+        la.k0 = k0;
+        la.n_shots = cnt;
+        la.n_blocks = n_blocks;
+        la.last_nz = last_nz;
+        la.total = total;
+        la.incl = d_incl;
+        la.excl = d_excl;
+        la.shots = d_shots;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sample_locate), (unsigned)((cnt + kBlock - 1) / kBlock), kBlock, 0,
+                           e->stream, la);
+        rc = e->check_launch("k_sample_locate");
+        if (rc) return rc;
         const unsigned g2 = (unsigned)((cnt + kBlock / 64 - 1) / (kBlock / 64));
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sample_find<T>), g2, kBlock, 0, e->stream, s, d_shots, cnt, spb, d_out);
         rc = e->check_launch("k_sample_find");
