@@ -98,9 +98,9 @@ template <typename T> __device__ inline T fmaT(T a, T b, T c) {
     else return __builtin_fmaf(a, b, c);
 }
 
-// Loads, stores and lane shuffles field by field: a 16-byte copy would make
-// the loop carry each vector as one <2 x i64> value, and every op that
-// updates a field would then rebuild (and copy) the whole register tuple.
+// Loads and stores field by field, so the op loop carries plain scalars (a
+// 16-byte copy into the Pack makes it carry <2 x i64> vectors that every op
+// unpacks and repacks).
 template <typename T> __device__ inline void pass_ld(const cx<T> *p, Pack<T> &r) {
     const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
     if constexpr (sizeof(T) == 8) {
