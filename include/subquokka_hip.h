@@ -104,7 +104,7 @@ typedef struct sq_config {
                                  VSWAPs relabelled and one set of qubit swaps at the end.
                                  1 = one launch per gate; subcircuits as VSWAP sweeps
                                  and one LDS tile pass per gate block */
-    int32_t pass_qbits;       /* register qubits per pass, 1..5 (0: 5 for SQ_C128, 4 for SQ_C64) */
+    int32_t pass_qbits;       /* register qubits per pass, 1..5 (0: 5) */
     int32_t host_tier;        /* out-of-core runner: SQ_HOST_AUTO or SQ_HOST_PAGEABLE */
 } sq_config;
 

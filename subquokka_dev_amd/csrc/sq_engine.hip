@@ -298,7 +298,7 @@ int create_engine(const sq_config *cfg, const char *const *files, int n_files, b
     e->sweep_mode = cfg->sweep_mode;
     // register qubits per pass: 4 for complex<double>; 3 for complex<float>,
     // whose 4-qubit pass exceeds the register file at two waves per SIMD
-    e->pass_qbits = cfg->pass_qbits > 0 ? cfg->pass_qbits : (cfg->dtype == SQ_C64 ? 4 : 5);
+    e->pass_qbits = cfg->pass_qbits > 0 ? cfg->pass_qbits : 5;
     if (hipSetDevice(e->device) != hipSuccess) { delete e; return SQ_ERR_HIP; }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { delete e; return SQ_ERR_HIP; }
     if (resident != 0) {
