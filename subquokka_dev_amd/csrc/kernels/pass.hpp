@@ -54,12 +54,27 @@ template <typename T> struct alignas(sizeof(T) == 8 ? 128 : 64) PassOp {
 };
 typedef int pass_v16i __attribute__((ext_vector_type(16)));
 
+// Address spaces of the device build: 1 is global memory, 4 constant memory.
+// The host build of these kernels (the CPU emulation) has one flat space.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SQK_AS_GLOBAL __attribute__((address_space(1)))
+#define SQK_AS_CONST __attribute__((address_space(4)))
+#else
+#define SQK_AS_GLOBAL
+#define SQK_AS_CONST
+#endif
+typedef SQK_AS_GLOBAL v4f pass_gv4f;
+
 // The whole record as dwords, read with one batch of aligned scalar loads.
 template <typename T> struct PassRec {
     static constexpr int NV = sizeof(PassOp<T>) / 64;
     pass_v16i w[NV];
     __device__ explicit PassRec(const PassOp<T> *p) {
-        const pass_v16i *q = reinterpret_cast<const pass_v16i *>(p);
+        // constant memory: the program is read-only while a pass runs, so the
+        // records stay scalar loads whatever the compiler can prove about the
+        // state stores (which go through addresses rebuilt from integers)
+        typedef SQK_AS_CONST const pass_v16i cpv;
+        cpv *q = (cpv *)p;
 #pragma unroll
         for (int i = 0; i < NV; i++) w[i] = q[i];
     }
@@ -100,9 +115,10 @@ template <typename T> __device__ inline T fmaT(T a, T b, T c) {
 
 // Loads and stores field by field, so the op loop carries plain scalars (a
 // 16-byte copy into the Pack makes it carry <2 x i64> vectors that every op
-// unpacks and repacks).
-template <typename T> __device__ inline void pass_ld(const cx<T> *p, Pack<T> &r) {
-    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
+// unpacks and repacks).  One 16-byte slot per lane: a cx<double> or two
+// cx<float>.
+template <typename T> __device__ inline void pass_ld(const pass_gv4f *p, Pack<T> &r) {
+    const v4f v = __builtin_nontemporal_load(p);
     if constexpr (sizeof(T) == 8) {
         typedef double v2d __attribute__((ext_vector_type(2)));
         const v2d d = __builtin_bit_cast(v2d, v);
@@ -112,7 +128,7 @@ template <typename T> __device__ inline void pass_ld(const cx<T> *p, Pack<T> &r)
         r.e[0].re = v.x; r.e[0].im = v.y; r.e[1].re = v.z; r.e[1].im = v.w;
     }
 }
-template <typename T> __device__ inline void pass_st(cx<T> *p, const Pack<T> &r) {
+template <typename T> __device__ inline void pass_st(pass_gv4f *p, const Pack<T> &r) {
     v4f v;
     if constexpr (sizeof(T) == 8) {
         typedef double v2d __attribute__((ext_vector_type(2)));
@@ -123,8 +139,25 @@ template <typename T> __device__ inline void pass_st(cx<T> *p, const Pack<T> &r)
     } else {
         v.x = r.e[0].re; v.y = r.e[0].im; v.z = r.e[1].re; v.w = r.e[1].im;
     }
-    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(p));
+    __builtin_nontemporal_store(v, p);
 }
+
+// Start of register row r of a tile, as a wave-uniform global address.  It
+// lives in scalar registers, so each of the 2^KH loads and stores takes it
+// plus one shared 32-bit lane offset, instead of a 64-bit vector address per
+// row kept live across the op loop (2^KH x 2 registers: at KH = 5 those
+// registers are what held the kernel to two waves per SIMD; it runs three).
+template <typename T, int KH>
+__device__ inline pass_gv4f *row_ptr(cx<T> *s, uint64_t tb, const uint64_t (&hb)[KH], int r) {
+    uint64_t off = tb;
+#pragma unroll
+    for (int j = 0; j < KH; j++) if ((r >> j) & 1) off += hb[j];
+    const uint64_t p = reinterpret_cast<uint64_t>(s + off);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)p);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(p >> 32));
+    return (pass_gv4f *)((uint64_t)lo | ((uint64_t)hi << 32));
+}
+
 // Cross-lane exchanges without the LDS.
 // Lane bits 0 and 1: one DPP quad_perm per dword.  Lane bits 2 and 3: two DPP
 // moves that both read the source -- row_shl by 4 (8) written only to the
@@ -336,7 +369,13 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
     // the wave index is uniform; readfirstlane lets the compiler know, so the
     // tile loop and every branch on tile bits are scalar branches, not
     // divergent regions that merge the state registers under exec masks
-    const uint64_t wave = (uint64_t)blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // Workgroups are dealt to the 8 XCDs in turn; renumbering them gives each
+    // XCD one contiguous eighth of the tiles.  This keeps the access pattern
+    // even when the register qubits are all high ones (32 rows a MiB or more
+    // apart), where the dealt order ran up to 1.35x slower.
+    uint64_t bid = blockIdx.x;
+    if ((gridDim.x & 7) == 0) bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
+    const uint64_t wave = bid * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t n_waves = (uint64_t)gridDim.x * (kBlock / 64);
 
     uint64_t hb[KH];
@@ -351,12 +390,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
 
         Pack<T> v[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            uint64_t off = base;
-#pragma unroll
-            for (int j = 0; j < KH; j++) if ((r >> j) & 1) off += hb[j];
-            pass_ld(s + off, v[r]);
-        }
+        for (int r = 0; r < R; r++) pass_ld<T>(row_ptr<T, KH>(s, tb, hb, r) + lane, v[r]);
 
         for (int op = 0; op < a.n_ops; op++) {
             // every field is read up front: one batch of scalar loads, one wait
@@ -473,12 +507,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
         }
 
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            uint64_t off = base;
-#pragma unroll
-            for (int j = 0; j < KH; j++) if ((r >> j) & 1) off += hb[j];
-            pass_st(s + off, v[r]);
-        }
+        for (int r = 0; r < R; r++) pass_st<T>(row_ptr<T, KH>(s, tb, hb, r) + lane, v[r]);
     }
 }
 
