@@ -24,7 +24,8 @@ struct PlainLaunch {
     int gate = -1;                 // single-gate step: index into the lowered list
     int op_begin = 0, n_ops = 0;   // merged step: slice of the uploaded ops
     double flops_per_amp = 0;      // algorithmic real flops per amplitude
-    int hpos[5] = {};              // register qubits of the pass
+    int hpos[5] = {};              // register qubits of the pass (virtual positions)
+    int lh[3] = {};                // qubits in the pass's top three lane positions
 };
 struct PlainProgram {
     int n = 0, kh = 0;
@@ -65,7 +66,8 @@ struct sq_engine {
     void *d_passprog = nullptr;     // register-pass program of the plain sweep
     size_t passprog_cap = 0;
     int sweep_mode = 0;             // 0: register passes, 1: one launch per gate
-    int pass_qbits = 4;             // register qubits per pass (1..4; set by sq_create)
+    int pass_qbits = 5;             // register qubits per pass (1..5; set by sq_create)
+    int lane_moves = 3;             // top lane positions a pass may give to other qubits (0..3)
 
     // out-of-core runner (resident_qbit > 0): state outside HBM
     int res_qbit = 0;

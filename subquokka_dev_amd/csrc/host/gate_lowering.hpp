@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <vector>
 #include <cstdint>
 #include <string>
 
@@ -152,6 +153,45 @@ inline int lower_gate(const sq_gate &g, int n_qubits, LoweredGate &out, std::str
         return SQ_ERR_UNKNOWN_GATE;
     }
     return SQ_OK;
+}
+
+// The gate with its targets mapped through sigma (a permutation of qubits:
+// the layout after VSWAP data moves, or a pass's virtual positions).  A two-qubit gate keeps its internal basis
+// (local bit j <-> j-th smallest target), so when the mapping reverses the
+// order of its targets, its matrix or diagonal is permuted to match.
+inline void relabel(LoweredGate &g, const std::vector<int> &sigma) {
+    if (g.cls == OpClass::Perm) {
+        for (int i = 0; i < 2 * g.n_pairs; i++) g.t[i] = sigma[g.t[i]];
+        return;
+    }
+    if (g.k == 1) {
+        g.t[0] = sigma[g.t[0]];
+        return;
+    }
+    const int a = sigma[g.t[0]], b = sigma[g.t[1]];
+    if (a < b) {
+        g.t[0] = a;
+        g.t[1] = b;
+        return;
+    }
+    g.t[0] = b;
+    g.t[1] = a;
+    auto sw = [](int l) { return ((l & 1) << 1) | ((l >> 1) & 1); };
+    if (g.cls == OpClass::Dense) {
+        std::complex<double> m[16];
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++) m[sw(r) * 4 + sw(c)] = g.m[r * 4 + c];
+        std::copy(m, m + 16, g.m);
+    } else {
+        std::complex<double> d[4];
+        uint32_t act = 0;
+        for (int l = 0; l < 4; l++) {
+            d[sw(l)] = g.m[l];
+            act |= ((g.diag_active >> l) & 1u) << sw(l);
+        }
+        std::copy(d, d + 4, g.m);
+        g.diag_active = act;
+    }
 }
 
 }  // namespace sq

@@ -161,12 +161,74 @@ int count_pass(const std::vector<sq_gate> &gates, const std::vector<int> &pendin
 
 }  // namespace
 
+namespace {
+
+// Splits a chosen qubit set between the register slots (at most kh) and the
+// movable lane positions [lo, lo + nlh): qubits already in a lane position
+// stay there; when more than kh others remain, those with the fewest dense
+// gates (lane gates cost more) and at most lh_max take the free lane
+// positions.  Returns false when the set does not fit.
+bool assign_lanes(const std::vector<int> &chosen, const std::vector<int> &dense_cnt, int lo, int nlh, int kh,
+                  int lh_max, std::vector<int> *regs, std::vector<int> *lanes) {
+    std::vector<int> lane(nlh, -1), rest;
+    for (int q : chosen) {
+        if (q >= lo && q < lo + nlh) lane[q - lo] = q;
+        else rest.push_back(q);
+    }
+    int need = (int)rest.size() - kh;
+    if (need > 0) {
+        std::vector<int> cand;
+        for (int q : rest) if (q <= lh_max) cand.push_back(q);
+        std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) { return dense_cnt[x] < dense_cnt[y]; });
+        int free_slots = 0;
+        for (int k = 0; k < nlh; k++) free_slots += lane[k] < 0;
+        if ((int)cand.size() < need || free_slots < need) return false;
+        std::vector<char> moved(dense_cnt.size(), 0);
+        for (int i = 0, k = 0; i < need; i++) {
+            while (lane[k] >= 0) k++;
+            lane[k] = cand[i];
+            moved[cand[i]] = 1;
+        }
+        std::vector<int> keep;
+        for (int q : rest) if (!moved[q]) keep.push_back(q);
+        rest.swap(keep);
+    }
+    if (regs) *regs = rest;
+    if (lanes) {
+        // a free position keeps its own qubit unless that qubit is a register
+        std::vector<char> used(dense_cnt.size(), 0);
+        for (int q : rest) used[q] = 1;
+        for (int k = 0; k < nlh; k++) if (lane[k] >= 0) used[lane[k]] = 1;
+        for (int k = 0; k < nlh; k++) {
+            if (lane[k] >= 0) continue;
+            if (!used[lo + k]) { lane[k] = lo + k; used[lo + k] = 1; }
+        }
+        // positions whose own qubit became a register take an untouched qubit
+        int pad = lo + nlh;
+        for (int k = 0; k < nlh; k++) {
+            if (lane[k] >= 0) continue;
+            while (pad < (int)used.size() && (used[pad] || pad > lh_max)) pad++;
+            if (pad >= (int)used.size()) return false;
+            lane[k] = pad;
+            used[pad] = 1;
+        }
+        *lanes = lane;
+    }
+    return true;
+}
+
+}  // namespace
+
 std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, const std::vector<char> &mergeable,
-                                            const std::vector<char> &anywhere, int n, int low, int kh) {
+                                            const std::vector<char> &anywhere, int n, int low, int kh, int nlh,
+                                            int lh_max) {
     std::vector<SweepStep> steps;
     std::vector<int> pending(gates.size());
     for (size_t i = 0; i < gates.size(); i++) pending[i] = (int)i;
     std::vector<char> in(n), blocked(n);
+    if (nlh < 0 || nlh > low) nlh = 0;
+    const int fixed = std::min(low - nlh, n);
+    const int lo = low - nlh;
     while (!pending.empty()) {
         const int g0 = pending.front();
         if (!mergeable[g0]) {
@@ -177,13 +239,25 @@ std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, c
             continue;
         }
         std::fill(in.begin(), in.end(), 0);
-        for (int q = 0; q < std::min(low, n); q++) in[q] = 1;
-        std::vector<int> regs;
+        for (int q = 0; q < fixed; q++) in[q] = 1;
+        std::vector<int> chosen;
+        if (nlh == 0) {
+            for (int q = fixed; q < std::min(low, n); q++) in[q] = 1;
+        }
+        // dense gates per qubit among the pending ones, for the lane choice
+        std::vector<int> dense_cnt(n, 0);
         int best = count_pass(gates, pending, in, mergeable, anywhere, blocked, nullptr);
-        while ((int)regs.size() < kh) {
+        const int picks = kh + nlh;
+        while ((int)chosen.size() < picks) {
             int best_q = -1, best_c = best;
-            for (int q = low; q < n; q++) {
+            for (int q = (nlh ? lo : low); q < n; q++) {
                 if (in[q]) continue;
+                if (nlh) {
+                    chosen.push_back(q);
+                    const bool fits = assign_lanes(chosen, dense_cnt, lo, nlh, kh, lh_max, nullptr, nullptr);
+                    chosen.pop_back();
+                    if (!fits) continue;
+                }
                 in[q] = 1;
                 const int c = count_pass(gates, pending, in, mergeable, anywhere, blocked, nullptr);
                 in[q] = 0;
@@ -191,7 +265,7 @@ std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, c
             }
             if (best_q < 0) break;
             in[best_q] = 1;
-            regs.push_back(best_q);
+            chosen.push_back(best_q);
             best = best_c;
         }
         SweepStep st;
@@ -201,8 +275,19 @@ std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, c
             st.merged = false;
             st.gates.push_back(g0);
         }
-        std::sort(regs.begin(), regs.end());
-        st.regs = regs;
+        if (nlh) {
+            for (int gi : st.gates)
+                if (!anywhere[gi])
+                    for (int i = 0; i < gates[gi].nq; i++) dense_cnt[gates[gi].q[i]]++;
+            if (!assign_lanes(chosen, dense_cnt, lo, nlh, kh, lh_max, &st.regs, &st.lanes)) {
+                // unreachable: every chosen set was checked; fall back to one gate
+                st = SweepStep{};
+                st.gates.push_back(g0);
+            }
+        } else {
+            st.regs = chosen;
+        }
+        std::sort(st.regs.begin(), st.regs.end());
         std::vector<int> rest;
         size_t k = 0;
         for (int gi : pending) {

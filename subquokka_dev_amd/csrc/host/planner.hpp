@@ -30,6 +30,9 @@ std::vector<PlanPass> plan_passes(const std::vector<sq_gate> &gates, int n, int 
 // Plain-sweep schedule of register passes.  A merged step applies its gates
 // in one read and one write of the state with the qubits [0, low) plus at
 // most kh register qubits resident; any other step is one gate on its own.
+// With nlh > 0 the top nlh of those low positions, [low - nlh, low), may hold
+// other qubits instead (at most lh_max): the step then has [0, low - nlh)
+// plus up to kh + nlh chosen qubits resident.
 //   mergeable[i]: gate i may run inside a register pass;
 //   anywhere[i]:  gate i may run whatever the pass's qubit set (diagonal
 //                 gates: the phase needs only the amplitude's index).
@@ -39,9 +42,11 @@ struct SweepStep {
     bool merged = false;
     std::vector<int> gates;      // indices into the input, in execution order
     std::vector<int> regs;       // register qubits of a merged step, ascending
+    std::vector<int> lanes;      // nlh > 0: qubit in position low - nlh + k (itself: unchanged)
 };
 std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, const std::vector<char> &mergeable,
-                                            const std::vector<char> &anywhere, int n, int low, int kh);
+                                            const std::vector<char> &anywhere, int n, int low, int kh,
+                                            int nlh = 0, int lh_max = -1);
 
 // Reference subcircuit text (finder.cpp:69-333 format) for a chunk of
 // chunk_qbit low qubits: gate blocks with relabelled targets separated by

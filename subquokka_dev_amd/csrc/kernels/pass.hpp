@@ -93,12 +93,19 @@ template <typename T> struct PassRec {
 };
 
 constexpr int kPassMaxKH = 5;
+constexpr int kPassLaneHi = 3;   // top lane bits a pass may give to other qubits
 
 template <typename T> struct PassArgs {
     const PassOp<T> *ops;
     uint64_t n_tiles;        // wave tiles = 2^(n - VB - 6 - KH)
     int n_ops;
     int hpos[kPassMaxKH];    // register qubits, ascending, all >= VB + 6
+    // The kernel works on a virtual index: its lane bits VB+3..VB+5 hold the
+    // qubits at physical bits lh[0..2].  Virtual bit VB+3+k and physical bit
+    // lh[k] trade places when a row is addressed (lh[k] = VB+3+k: no trade);
+    // lh[k] - VB <= 27, so a lane's offset stays below 4 GiB.  Everything
+    // else (register qubits, tile bits, op targets) is in virtual positions.
+    int lh[kPassLaneHi];
 };
 
 // In-place updates.  The op loop carries the state in fixed registers, so
@@ -378,19 +385,36 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
     const uint64_t wave = bid * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t n_waves = (uint64_t)gridDim.x * (kBlock / 64);
 
+    // virtual -> physical: swap bit VB+3+k with bit lh[k]
+    auto phys = [&](uint64_t x) {
+#pragma unroll
+        for (int k = 0; k < kPassLaneHi; k++) {
+            const int u = VB + 3 + k, w = a.lh[k];
+            const uint64_t d = ((x >> u) ^ (x >> w)) & 1;
+            x ^= (d << u) | (d << w);
+        }
+        return x;
+    };
     uint64_t hb[KH];
 #pragma unroll
-    for (int j = 0; j < KH; j++) hb[j] = 1ULL << a.hpos[j];
+    for (int j = 0; j < KH; j++) hb[j] = phys(1ULL << a.hpos[j]);
+    // the lane's 16-byte slot within a row: lane bits 0-2 stay in place, lane
+    // bit 3+k goes to slot bit lh[k] - VB (below 2^28 slots: a 32-bit offset)
+    int lane_slot = lane & 7;
+#pragma unroll
+    for (int k = 0; k < kPassLaneHi; k++) lane_slot |= ((lane >> (3 + k)) & 1) << (a.lh[k] - VB);
+    lane_slot &= (1 << 28) - 1;
 
     for (uint64_t t = wave; t < a.n_tiles; t += n_waves) {
         uint64_t tb = t << S;   // tile bits only (uniform)
 #pragma unroll
         for (int j = 0; j < KH; j++) tb = insert_zero(tb, a.hpos[j]);
-        const uint64_t base = tb + ((uint64_t)lane << VB);
+        const uint64_t base = tb + ((uint64_t)lane << VB);   // virtual index of register 0
+        const uint64_t ptb = phys(tb);
 
         Pack<T> v[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) pass_ld<T>(row_ptr<T, KH>(s, tb, hb, r) + lane, v[r]);
+        for (int r = 0; r < R; r++) pass_ld<T>(row_ptr<T, KH>(s, ptb, hb, r) + lane_slot, v[r]);
 
         for (int op = 0; op < a.n_ops; op++) {
             // every field is read up front: one batch of scalar loads, one wait
@@ -507,7 +531,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
         }
 
 #pragma unroll
-        for (int r = 0; r < R; r++) pass_st<T>(row_ptr<T, KH>(s, tb, hb, r) + lane, v[r]);
+        for (int r = 0; r < R; r++) pass_st<T>(row_ptr<T, KH>(s, ptb, hb, r) + lane_slot, v[r]);
     }
 }
 

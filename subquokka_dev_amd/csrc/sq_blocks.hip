@@ -114,45 +114,6 @@ void encode_op(const LoweredGate &g, const std::vector<int> &q, std::vector<Tile
     ops.push_back(o);
 }
 
-// The gate as it acts after the data moves the VSWAPs so far would have made:
-// targets mapped through sigma.  A two-qubit gate keeps its internal basis
-// (local bit j <-> j-th smallest target), so when the mapping reverses the
-// order of its targets, its matrix or diagonal is permuted to match.
-void relabel(LoweredGate &g, const std::vector<int> &sigma) {
-    if (g.cls == OpClass::Perm) {
-        for (int i = 0; i < 2 * g.n_pairs; i++) g.t[i] = sigma[g.t[i]];
-        return;
-    }
-    if (g.k == 1) {
-        g.t[0] = sigma[g.t[0]];
-        return;
-    }
-    const int a = sigma[g.t[0]], b = sigma[g.t[1]];
-    if (a < b) {
-        g.t[0] = a;
-        g.t[1] = b;
-        return;
-    }
-    g.t[0] = b;
-    g.t[1] = a;
-    auto sw = [](int l) { return ((l & 1) << 1) | ((l >> 1) & 1); };
-    if (g.cls == OpClass::Dense) {
-        std::complex<double> m[16];
-        for (int r = 0; r < 4; r++)
-            for (int c = 0; c < 4; c++) m[sw(r) * 4 + sw(c)] = g.m[r * 4 + c];
-        std::copy(m, m + 16, g.m);
-    } else {
-        std::complex<double> d[4];
-        uint32_t act = 0;
-        for (int l = 0; l < 4; l++) {
-            d[sw(l)] = g.m[l];
-            act |= ((g.diag_active >> l) & 1u) << sw(l);
-        }
-        std::copy(d, d + 4, g.m);
-        g.diag_active = act;
-    }
-}
-
 int run_blocks_mapped(sq_engine *e, const sq_gate *gates, const std::vector<LoweredGate> &low,
                       const uint32_t *block_sizes, uint64_t n_blocks) {
     const int n = e->n;
@@ -170,7 +131,7 @@ int run_blocks_mapped(sq_engine *e, const sq_gate *gates, const std::vector<Lowe
                 continue;
             }
             LoweredGate lg = low[i];
-            relabel(lg, sigma);
+            sq::relabel(lg, sigma);
             sq_gate mg = g;
             for (int j = 0; j < g.nq; j++) mg.q[j] = sigma[g.q[j]];
             flat.push_back(mg);

@@ -28,6 +28,7 @@ using namespace sqk;
 namespace {
 
 static_assert(kPassMaxKH <= 5, "PlainLaunch holds five register qubits");
+static_assert(kPassLaneHi == 3, "PlainLaunch holds three lane positions");
 
 template <typename T>
 void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vector<PassOp<T>> &ops) {
@@ -140,7 +141,11 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
         if (g.cls == OpClass::Diag && g.k <= 2) mergeable[i] = anywhere[i] = 1;
         else if (g.cls == OpClass::Dense && g.k == 1) mergeable[i] = 1;
     }
-    const auto steps = sq::plan_register_passes(gates, mergeable, anywhere, n, S, kh);
+    // the top three lane positions may hold any qubit up to bit 27 of the
+    // 16-byte slot index (a lane's offset stays a 32-bit one)
+    const int moves = std::max(0, std::min(e->lane_moves, kPassLaneHi));
+    const auto steps = sq::plan_register_passes(gates, mergeable, anywhere, n, S, kh, moves,
+                                                std::min(n - 1, 27 + VB));
 
     std::vector<PassOp<T>> ops;
     for (const auto &st : steps) {
@@ -154,16 +159,30 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
             }
             continue;
         }
-        // exactly kh register qubits: pad with qubits the pass does not touch
-        std::vector<int> regs = st.regs;
-        for (int q = S; q < n && (int)regs.size() < kh; q++)
-            if (std::find(regs.begin(), regs.end(), q) == regs.end()) regs.push_back(q);
-        std::sort(regs.begin(), regs.end());
+        // Virtual positions: lane position S-3+k holds qubit st.lanes[k], and
+        // that qubit's own position holds the one it displaced.
+        std::vector<int> vpos(n);
+        for (int q = 0; q < n; q++) vpos[q] = q;
         PlainLaunch l;
+        for (int k = 0; k < kPassLaneHi; k++) {
+            const int u = S - kPassLaneHi + k;
+            const int off = k - (kPassLaneHi - moves);   // st.lanes covers the top `moves` positions
+            const int w = off >= 0 ? st.lanes[off] : u;
+            l.lh[k] = w;
+            if (w != u) std::swap(vpos[u], vpos[w]);
+        }
+        // exactly kh register qubits: pad with qubits outside the lane positions
+        std::vector<int> regs;
+        for (int q : st.regs) regs.push_back(vpos[q]);
+        for (int q = 0; q < n && (int)regs.size() < kh; q++)
+            if (vpos[q] >= S && std::find(regs.begin(), regs.end(), vpos[q]) == regs.end()) regs.push_back(vpos[q]);
+        std::sort(regs.begin(), regs.end());
         l.merged = true;
         l.op_begin = (int)ops.size();
         for (int gi : st.gates) {
-            encode_pass_op<T>(low[gi], regs, ops);
+            LoweredGate g = low[gi];
+            sq::relabel(g, vpos);
+            encode_pass_op<T>(g, regs, ops);
             // dense 2x2 row: two complex products and a sum; diagonal: one product
             l.flops_per_amp += low[gi].cls == OpClass::Dense ? 14.0 : 6.0;
         }
@@ -222,6 +241,7 @@ int run_plain_t(sq_engine *e, const PlainProgram &prog, const std::vector<Lowere
         a.n_ops = l.n_ops;
         a.n_tiles = 1ULL << (n - S - kh);
         for (int j = 0; j < kh; j++) a.hpos[j] = l.hpos[j];
+        for (int k = 0; k < kPassLaneHi; k++) a.lh[k] = l.lh[k];
         int slot = e->stat_begin(nm, bytes, l.flops_per_amp * std::ldexp(1.0, n));
         int rc;
         switch (kh) {

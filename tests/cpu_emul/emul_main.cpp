@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "subquokka_hip.h"
+#include "engine_internal.hpp"
 
 namespace sqk { alignas(16) unsigned char lds_raw[1 << 16]; }
 
@@ -56,6 +57,11 @@ int main(int argc, char **argv) {
         rc = sq_create(&cfg, &e);
     }
     if (rc) { std::fprintf(stderr, "create rc=%d\n", rc); return 4; }
+    // Cross-lane exchanges are not modelled here (one lane runs at a time), so
+    // register passes keep the lane qubits in place unless EMUL_LANE_MOVES asks
+    // otherwise (then only the addressing is checked, not the values).
+    e->lane_moves = 0;
+    if (const char *lm = std::getenv("EMUL_LANE_MOVES")) e->lane_moves = std::atoi(lm);
     if (mode == "adjoint") {   // chunk argument = layers; theta from a fixed formula
         const int layers = chunk;
         std::vector<double> theta((size_t)2 * n * layers), grad(theta.size());

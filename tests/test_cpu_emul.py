@@ -36,7 +36,8 @@ def emul():
     return EMUL
 
 
-def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None, tier=None, pass_qbits=None, sweep_mode=None):
+def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None, tier=None, pass_qbits=None, sweep_mode=None,
+             lane_moves=None):
     f = tmp_path / "c.txt"
     f.write_text(text)
     out = tmp_path / f"state_{tier or 'default'}.bin"
@@ -47,6 +48,8 @@ def run_case(emul, tmp_path, mode, n, chunk, dtype, text, seed=None, tier=None, 
         env["EMUL_PASS_QBITS"] = str(pass_qbits)
     if sweep_mode is not None:
         env["EMUL_SWEEP_MODE"] = str(sweep_mode)
+    if lane_moves is not None:
+        env["EMUL_LANE_MOVES"] = str(lane_moves)
     if tier is not None:
         env["EMUL_HOST_TIER"] = tier
         env["EMUL_FILES_DIR"] = str(tmp_path)
@@ -164,6 +167,19 @@ def register_pass_circuit(n, n_gates, seed, first_dense):
             qs = [rng.randrange(first_dense, n)]
         lines.append(gate_line(k, qs, rng, nrng))
     return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("moves", [1, 3])
+@pytest.mark.parametrize("kh", [2, 5])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [12, 14])
+def test_register_pass_lane_moves_in_bounds(emul, tmp_path, dtype, n, kh, moves):
+    # passes that hand the top lane positions to other qubits: every row and
+    # lane offset stays inside the state (AddressSanitizer).  The values are
+    # not checked here: lane exchanges are not modelled on the CPU; the GPU
+    # parity tests check them.
+    text = register_pass_circuit(n, 250, 3 * n, 0)
+    run_case(emul, tmp_path, "plain", n, 4, dtype, text, seed=n, pass_qbits=kh, lane_moves=moves)
 
 
 @pytest.mark.parametrize("kh", [4, 5])
