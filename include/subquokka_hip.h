@@ -232,7 +232,8 @@ int sq_gen_state(int32_t n_qubits, uint64_t seed, double *out);
  *   mode 3: mode 0 plus swaps that restore the qubit order at the end;
  *   mode 1: fused blocks of at most param qubits (fused_k);
  *   mode 2: the plain sweep's register-pass schedule for complex128 (qubits
- *           0-5 resident plus at most param register qubits per step). */
+ *           0-2 resident plus at most param + 3 chosen qubits per step: param
+ *           register qubits and three lane positions). */
 int sq_plan(const char *circuit_text, int32_t total_qbit, int32_t mode, int32_t param,
             char *buf, uint64_t cap, uint64_t *len);
 

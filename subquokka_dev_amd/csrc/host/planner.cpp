@@ -404,8 +404,9 @@ extern "C" int sq_plan(const char *circuit_text, int32_t n, int32_t mode, int32_
             for (int gi : p.gates) out += sq::gate_to_line(gates[gi]) + "\n";
         }
     } else if (mode == 2) {
-        // register passes of the complex128 plain sweep: lane qubits 0-5 plus
-        // at most param register qubits; other gates stand alone
+        // register passes of the complex128 plain sweep: qubits 0-2 plus at most
+        // param + 3 chosen qubits (param in registers, up to three in the top
+        // lane positions); other gates stand alone
         if (param < 1 || param > 5) return SQ_ERR_ARG;
         std::vector<char> mergeable(gates.size(), 0), anywhere(gates.size(), 0);
         for (size_t i = 0; i < gates.size(); i++) {
@@ -414,7 +415,7 @@ extern "C" int sq_plan(const char *circuit_text, int32_t n, int32_t mode, int32_
             if (lg.cls == sq::OpClass::Diag && lg.k <= 2) mergeable[i] = anywhere[i] = 1;
             else if (lg.cls == sq::OpClass::Dense && lg.k == 1) mergeable[i] = 1;
         }
-        for (auto &st : sq::plan_register_passes(gates, mergeable, anywhere, n, 6, param)) {
+        for (auto &st : sq::plan_register_passes(gates, mergeable, anywhere, n, 6, param, 3, std::min(n - 1, 27))) {
             out += std::to_string(st.gates.size()) + "\n";
             for (int gi : st.gates) out += sq::gate_to_line(gates[gi]) + "\n";
         }

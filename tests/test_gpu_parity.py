@@ -146,6 +146,34 @@ def test_register_passes_match_one_launch_per_gate(oracle, dtype, kh):
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("kh", [2, 5])
+def test_register_passes_with_lane_moves(oracle, dtype, kh):
+    # dense gates on more qubits than the register slots hold, with the top
+    # lane qubits idle in between: the planner hands those lane positions to
+    # other qubits (the pass swaps lane and address bits), and the gates on
+    # the displaced lane qubits run as register or tile-bit work
+    n = 18
+    lines = []
+    for layer in range(12):
+        qs = [3, 4, 5] if layer % 3 == 0 else list(range(6 + layer % 4, n, 2))
+        for q in qs:
+            lines.append(f"U1 {q} 0.6 0.0 0.0 0.8 0.8 0.0 0.0 -0.6")
+        lines.append(f"CP {layer % n} {(layer * 7 + 5) % n} 0.{layer + 1}")
+        lines.append(f"RZ {(layer * 5) % n} 0.3")
+    text = "\n".join(lines) + "\n" + random_circuit(n, 120, seed=40 + kh)
+    s0 = random_state(n, 40 + kh)
+    say(f"register passes lane moves n={n} kh={kh} {dtype}")
+    host = s0.astype(np.complex128 if dtype == "c128" else np.complex64)
+    a = sq.Engine(n, dtype=dtype, pass_qbits=kh)
+    try:
+        a.set_state(host)
+        a.apply_text(text)
+        check(a.get_state(np.complex128), oracle.run(s0, n, text), dtype, f"lane moves kh={kh}")
+    finally:
+        a.close()
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("n", [10, 11, 12])
 def test_register_passes_smallest_states(oracle, dtype, n):
     # n = lane bits + register qubits exactly (one wave tile) and just above

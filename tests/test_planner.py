@@ -99,9 +99,10 @@ DENSE1 = {"H", "X", "Y", "U1", "RX", "RY"}
 
 @pytest.mark.parametrize("n,kh,seed", [(10, 1, 1), (11, 2, 2), (13, 4, 3), (16, 4, 4), (16, 3, 5), (16, 5, 6), (12, 5, 7)])
 def test_register_pass_schedule(oracle, n, kh, seed):
-    # the plain sweep's schedule (sq_plan mode 2, complex128 lanes = qubits 0-5):
+    # the plain sweep's schedule (sq_plan mode 2, complex128: lane qubits 0-2
+    # fixed, three lane positions and kh register slots for chosen qubits):
     #   * a step of several gates holds only one-qubit dense and diagonal gates,
-    #     with its dense targets in qubits 0-5 plus at most kh other qubits;
+    #     with its dense targets in qubits 0-2 plus at most kh + 3 other qubits;
     #   * every qubit sees its gates in the input order;
     #   * so the oracle computes the same state from the schedule.
     text = sq.gen_circuit("random", n, 4, seed=seed) + random_circuit(n, 200, seed=seed)
@@ -124,13 +125,22 @@ def test_register_pass_schedule(oracle, n, kh, seed):
         merged += 1
         kinds = {l.split()[0] for l in blk}
         assert kinds <= DIAG | DENSE1, kinds
-        regs = {qubits_of(l)[0] for l in blk if l.split()[0] in DENSE1} - set(range(6))
-        assert len(regs) <= kh, (regs, kh)
+        chosen = {qubits_of(l)[0] for l in blk if l.split()[0] in DENSE1} - set(range(3))
+        assert len(chosen) <= kh + 3, (chosen, kh)
     assert merged > 0
     for q in range(n):
         assert [g for g in flat_n if q in g[1]] == [g for g in src_n if q in g[1]]
     s0 = random_state(n, seed)
     np.testing.assert_allclose(oracle.run(s0, n, "\n".join(flat) + "\n"), oracle.run(s0, n, text), rtol=0, atol=1e-12)
+
+
+def test_register_pass_schedule_lane_moves_cut_passes():
+    # the engine's schedule for the 30-qubit, 900-gate random-layer circuit:
+    # choosing the top three lane qubits per pass cuts 36 passes to 25
+    text = sq.gen_circuit("random", 30, 20, seed=2)
+    blocks = blocks_of(sq.plan(text, 30, 2, 5))
+    assert sum(len(b) for b in blocks) == 900
+    assert len(blocks) <= 26, len(blocks)
 
 
 def test_register_pass_schedule_cuts_passes():
