@@ -217,6 +217,59 @@ bool assign_lanes(const std::vector<int> &chosen, const std::vector<int> &dense_
     return true;
 }
 
+// The other split: the |chosen| - kh lane-capable qubits with the fewest
+// dense gates take lane positions, even when that sends a qubit already in a
+// lane position to a register.  Returns false when it does not apply.
+bool assign_lanes_min_dense(const std::vector<int> &chosen, const std::vector<int> &dense_cnt, int lo, int nlh,
+                            int kh, int lh_max, std::vector<int> *regs, std::vector<int> *lanes) {
+    const int m = (int)chosen.size() - kh;
+    if (m <= 0 || m > nlh) return false;
+    auto in_lane_pos = [&](int q) { return q >= lo && q < lo + nlh; };
+    std::vector<int> cand;
+    for (int q : chosen) if (q <= lh_max || in_lane_pos(q)) cand.push_back(q);
+    if ((int)cand.size() < m) return false;
+    std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) {
+        if (dense_cnt[x] != dense_cnt[y]) return dense_cnt[x] < dense_cnt[y];
+        return in_lane_pos(x) && !in_lane_pos(y);
+    });
+    const int nq = (int)dense_cnt.size();
+    std::vector<char> is_lane(nq, 0), is_chosen(nq, 0), used(nq, 0);
+    for (int i = 0; i < m; i++) is_lane[cand[i]] = 1;
+    for (int q : chosen) is_chosen[q] = used[q] = 1;
+    std::vector<int> lane(nlh, -1), rest;
+    for (int q : chosen) {
+        if (!is_lane[q]) rest.push_back(q);
+        else if (in_lane_pos(q)) lane[q - lo] = q;
+    }
+    for (int q : chosen) {
+        if (!is_lane[q] || in_lane_pos(q)) continue;
+        int k = 0;
+        while (k < nlh && (lane[k] >= 0 || is_lane[lo + k])) k++;
+        if (k == nlh) return false;
+        lane[k] = q;
+    }
+    int pad = lo + nlh;
+    for (int k = 0; k < nlh; k++) {
+        if (lane[k] >= 0) continue;
+        if (!is_chosen[lo + k]) { lane[k] = lo + k; continue; }
+        while (pad < nq && (used[pad] || pad > lh_max)) pad++;
+        if (pad >= nq) return false;
+        lane[k] = pad;
+        used[pad] = 1;
+    }
+    *regs = rest;
+    *lanes = lane;
+    return true;
+}
+
+// Dense gates left in lane positions plus positions moved (each move splits
+// the rows into shorter pieces): both cost about the same per unit.
+int lane_cost(const std::vector<int> &lanes, const std::vector<int> &dense_cnt, int lo) {
+    int c = 0;
+    for (size_t k = 0; k < lanes.size(); k++) c += dense_cnt[lanes[k]] + (lanes[k] != lo + (int)k);
+    return c;
+}
+
 }  // namespace
 
 std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, const std::vector<char> &mergeable,
@@ -283,6 +336,13 @@ std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, c
                 // unreachable: every chosen set was checked; fall back to one gate
                 st = SweepStep{};
                 st.gates.push_back(g0);
+            } else {
+                std::vector<int> r2, l2;
+                if (assign_lanes_min_dense(chosen, dense_cnt, lo, nlh, kh, lh_max, &r2, &l2) &&
+                    lane_cost(l2, dense_cnt, lo) < lane_cost(st.lanes, dense_cnt, lo)) {
+                    st.regs = r2;
+                    st.lanes = l2;
+                }
             }
         } else {
             st.regs = chosen;
