@@ -26,7 +26,8 @@
 namespace sqk {
 
 enum PassOpKind : int32_t {
-    PO_DENSE_REG = 0,    // a = register slot of the target
+    PO_DENSE_REG = 0,    // a = register slot of the target; c = 15 when all four coefficients
+                         //   are real, 1 when m0 is, else 0 (the kernel leaves out zero terms)
     PO_DENSE_LANE = 1,   // a = lane bit of the target (0..5)
     PO_DENSE_VEC = 2,    // target inside the 16-byte vector (complex<float>, bit 0)
     PO_DIAG_FREE = 3,    // diagonal, no bit in a register slot: a, b = global bits (b < 0: one-qubit);
@@ -272,6 +273,35 @@ __device__ inline void ip_pair(cx<T> &x, cx<T> &y, const cx<T> m0, const cx<T> m
     fma_into(y.im, m3.re, pyi);
 }
 
+// The same update when coefficients are known to be real (imaginary part
+// exactly 0): RM = 1, m0 real (U3 and the U1 form of it); RM = 15, all four
+// real (H, X, RY, ...).  The terms left out are exact zeros, so the results
+// equal ip_pair's (up to the sign of a zero): 14 and 8 FP64 operations per
+// pair instead of 16.
+template <int RM, typename T>
+__device__ inline void ip_pair_real(cx<T> &x, cx<T> &y, const cx<T> m0, const cx<T> m1, const cx<T> m2,
+                                    const cx<T> m3) {
+    if constexpr (RM == 15) {
+        const T pxr = m1.re * y.re, pxi = m1.re * y.im;
+        const T pyr = m2.re * x.re, pyi = m2.re * x.im;
+        fma_into(x.re, m0.re, pxr);
+        fma_into(x.im, m0.re, pxi);
+        fma_into(y.re, m3.re, pyr);
+        fma_into(y.im, m3.re, pyi);
+    } else if constexpr (RM == 1) {
+        const T pxr = fmaT(m1.re, y.re, -(m1.im * y.im));
+        const T pxi = fmaT(m1.re, y.im, m1.im * y.re);
+        const T pyr = fmaT(m2.re, x.re, fmaT(-m2.im, x.im, -(m3.im * y.im)));
+        const T pyi = fmaT(m2.re, x.im, fmaT(m2.im, x.re, m3.im * y.re));
+        fma_into(x.re, m0.re, pxr);
+        fma_into(x.im, m0.re, pxi);
+        fma_into(y.re, m3.re, pyr);
+        fma_into(y.im, m3.re, pyi);
+    } else {
+        ip_pair(x, y, m0, m1, m2, m3);
+    }
+}
+
 // x <- ms x + mp p (p: the partner's value, already in other registers).
 template <typename T> __device__ inline void ip_lane(cx<T> &x, const cx<T> p, const cx<T> ms, const cx<T> mp) {
     const T pr = fmaT(mp.re, p.re, fmaT(-mp.im, p.im, -(ms.im * x.im)));
@@ -281,14 +311,27 @@ template <typename T> __device__ inline void ip_lane(cx<T> &x, const cx<T> p, co
 }
 
 // One dense 2x2 on the register pairs (r, r | 2^J).
-template <typename T, int J, int R, int E>
+template <typename T, int J, int R, int E, int RM = 0>
 __device__ inline void pass_dense_reg(Pack<T> (&v)[R], const cx<T> m0, const cx<T> m1, const cx<T> m2, const cx<T> m3) {
 #pragma unroll
     for (int r = 0; r < R; r++) {
         if ((r >> J) & 1) continue;
         const int q = r | (1 << J);
 #pragma unroll
-        for (int e = 0; e < E; e++) ip_pair(v[r].e[e], v[q].e[e], m0, m1, m2, m3);
+        for (int e = 0; e < E; e++) ip_pair_real<RM>(v[r].e[e], v[q].e[e], m0, m1, m2, m3);
+    }
+}
+
+template <typename T, int KH, int R, int E, int RM>
+__device__ inline void pass_dense_reg_at(Pack<T> (&v)[R], int j, const cx<T> m0, const cx<T> m1, const cx<T> m2,
+                                         const cx<T> m3) {
+    switch (j) {
+    case 0: pass_dense_reg<T, 0, R, E, RM>(v, m0, m1, m2, m3); break;
+    case 1: if constexpr (KH > 1) pass_dense_reg<T, 1, R, E, RM>(v, m0, m1, m2, m3); break;
+    case 2: if constexpr (KH > 2) pass_dense_reg<T, 2, R, E, RM>(v, m0, m1, m2, m3); break;
+    case 3: if constexpr (KH > 3) pass_dense_reg<T, 3, R, E, RM>(v, m0, m1, m2, m3); break;
+    case 4: if constexpr (KH > 4) pass_dense_reg<T, 4, R, E, RM>(v, m0, m1, m2, m3); break;
+    default: break;
     }
 }
 
@@ -424,15 +467,11 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
             const Coef4<T> c = {rec.coef(0), rec.coef(1), rec.coef(2), rec.coef(3)};
             switch (o.kind) {
             case PO_DENSE_REG: {
+                // o.c: 15 all coefficients real, 1 m0 real, 0 general
                 const cx<T> m0 = c.c0, m1 = c.c1, m2 = c.c2, m3 = c.c3;
-                switch (o.a) {
-                case 0: pass_dense_reg<T, 0, R, E>(v, m0, m1, m2, m3); break;
-                case 1: if constexpr (KH > 1) pass_dense_reg<T, 1, R, E>(v, m0, m1, m2, m3); break;
-                case 2: if constexpr (KH > 2) pass_dense_reg<T, 2, R, E>(v, m0, m1, m2, m3); break;
-                case 3: if constexpr (KH > 3) pass_dense_reg<T, 3, R, E>(v, m0, m1, m2, m3); break;
-                case 4: if constexpr (KH > 4) pass_dense_reg<T, 4, R, E>(v, m0, m1, m2, m3); break;
-                default: break;
-                }
+                if (o.c == 15) pass_dense_reg_at<T, KH, R, E, 15>(v, o.a, m0, m1, m2, m3);
+                else if (o.c == 1) pass_dense_reg_at<T, KH, R, E, 1>(v, o.a, m0, m1, m2, m3);
+                else pass_dense_reg_at<T, KH, R, E, 0>(v, o.a, m0, m1, m2, m3);
                 break; }
             case PO_DENSE_LANE: {
                 const bool hi = (lane >> o.a) & 1;

@@ -53,6 +53,10 @@ void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vec
         } else {
             o.kind = PO_DENSE_REG;
             o.a = slot(t);
+            // imaginary parts that are exactly zero as stored (in T)
+            bool all_real = true;
+            for (int i = 0; i < 4; i++) all_real = all_real && o.coef[i].im == (T)0;
+            o.c = all_real ? 15 : (o.coef[0].im == (T)0 ? 1 : 0);
         }
         ops.push_back(o);
         return;

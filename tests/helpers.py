@@ -6,6 +6,7 @@ The oracle (oracle/libsq_oracle.so) and the reference harness
 check and bench.py's cpu_baseline leg load them.
 """
 import ctypes
+import cmath
 import math
 import os
 import random
@@ -112,6 +113,11 @@ def gate_line(kind, qs, rng_py, nrng):
         return f"{kind} {qs[0]}"
     if kind in ("P", "RX", "RY", "RZ"):
         return f"{kind} {qs[0]} {rng_py.uniform(0, 2 * math.pi)!r}"
+    if kind == "U3":   # U3(theta, phi, lambda) written as U1: m0 = cos(theta/2) is real
+        th, ph, la = (rng_py.uniform(0, 2 * math.pi) for _ in range(3))
+        c, s = math.cos(th / 2), math.sin(th / 2)
+        u = [complex(c, 0.0), -cmath.exp(1j * la) * s, cmath.exp(1j * ph) * s, cmath.exp(1j * (ph + la)) * c]
+        return f"U1 {qs[0]} " + " ".join(f"{float(v.real)!r} {float(v.imag)!r}" for v in u)
     if kind == "U1":
         u = random_unitary(2, nrng)
         return f"U1 {qs[0]} " + " ".join(f"{float(v.real)!r} {float(v.imag)!r}" for v in u.flatten())

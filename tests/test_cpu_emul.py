@@ -158,7 +158,7 @@ def register_pass_circuit(n, n_gates, seed, first_dense):
     nrng = np.random.default_rng(seed)
     lines = []
     for _ in range(n_gates):
-        k = rng.choice(["H", "X", "Y", "U1", "RX", "RY", "Z", "P", "RZ", "CP", "RZZ", "CP", "RZZ"])
+        k = rng.choice(["H", "X", "Y", "U1", "U3", "RX", "RY", "Z", "P", "RZ", "CP", "RZZ", "CP", "RZZ"])
         if k in ("CP", "RZZ"):
             qs = sorted(rng.sample(range(n), 2))
         elif k in ("Z", "P", "RZ"):
