@@ -37,6 +37,12 @@ enum PassOpKind : int32_t {
                          //   coefficients e[(register bit << 1) | other bit]; c = 1 when the other
                          //   bit is a tile bit or absent (wave-uniform)
     PO_DIAG_REG2 = 5,    // diagonal, both bits in register slots a < b
+    PO_DENSE_REG_C = 6,  // complex<double> only: a controlled phase folded into the dense
+                         //   gate on register slot a that follows it: where the control
+                         //   bit is 1 the gate is U diag(1, ph), i.e. m1 and m3 times ph
+                         //   (coef[4]).  b = the control's position; active = 0: register
+                         //   slot b, 1: lane bit b - VB, 2: tile bit b.  c as PO_DENSE_REG
+                         //   (1: m0 real)
 };
 
 // 32-bit fields only: the op loop reads records with scalar (SMEM) loads,
@@ -51,7 +57,7 @@ template <typename T> struct alignas(sizeof(T) == 8 ? 128 : 64) PassOp {
     int32_t a, b, c;
     uint32_t active;     // diagonal: bit k set when coefficient k is not 1
     int32_t pad[3];
-    cx<T> coef[4];
+    cx<T> coef[sizeof(T) == 8 ? 5 : 4];   // complex<double>: coef[4] = PO_DENSE_REG_C's phase
 };
 typedef int pass_v16i __attribute__((ext_vector_type(16)));
 
@@ -95,6 +101,7 @@ template <typename T> struct PassRec {
 
 constexpr int kPassMaxKH = 5;
 constexpr int kPassLaneHi = 3;   // top lane bits a pass may give to other qubits
+constexpr int kPassFoldPhase = 4;   // PassOp<double>::coef slot of PO_DENSE_REG_C's phase
 
 template <typename T> struct PassArgs {
     const PassOp<T> *ops;
@@ -322,6 +329,38 @@ __device__ inline void pass_dense_reg(Pack<T> (&v)[R], const cx<T> m0, const cx<
     }
 }
 
+// PO_DENSE_REG_C with the control in register slot sj (wave-uniform): each
+// pair takes the coefficients of its control value, one uniform branch each.
+template <typename T, int J, int R, int E, int RM>
+__device__ inline void pass_dense_reg_ctl(Pack<T> (&v)[R], int sj, const cx<T> m0, const cx<T> m1, const cx<T> m2,
+                                          const cx<T> m3, const cx<T> m1p, const cx<T> m3p) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if ((r >> J) & 1) continue;
+        const int q = r | (1 << J);
+        if ((r >> sj) & 1) {
+#pragma unroll
+            for (int e = 0; e < E; e++) ip_pair_real<RM>(v[r].e[e], v[q].e[e], m0, m1p, m2, m3p);
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; e++) ip_pair_real<RM>(v[r].e[e], v[q].e[e], m0, m1, m2, m3);
+        }
+    }
+}
+
+template <typename T, int KH, int R, int E, int RM>
+__device__ inline void pass_dense_reg_ctl_at(Pack<T> (&v)[R], int j, int sj, const cx<T> m0, const cx<T> m1,
+                                             const cx<T> m2, const cx<T> m3, const cx<T> m1p, const cx<T> m3p) {
+    switch (j) {
+    case 0: pass_dense_reg_ctl<T, 0, R, E, RM>(v, sj, m0, m1, m2, m3, m1p, m3p); break;
+    case 1: if constexpr (KH > 1) pass_dense_reg_ctl<T, 1, R, E, RM>(v, sj, m0, m1, m2, m3, m1p, m3p); break;
+    case 2: if constexpr (KH > 2) pass_dense_reg_ctl<T, 2, R, E, RM>(v, sj, m0, m1, m2, m3, m1p, m3p); break;
+    case 3: if constexpr (KH > 3) pass_dense_reg_ctl<T, 3, R, E, RM>(v, sj, m0, m1, m2, m3, m1p, m3p); break;
+    case 4: if constexpr (KH > 4) pass_dense_reg_ctl<T, 4, R, E, RM>(v, sj, m0, m1, m2, m3, m1p, m3p); break;
+    default: break;
+    }
+}
+
 template <typename T, int KH, int R, int E, int RM>
 __device__ inline void pass_dense_reg_at(Pack<T> (&v)[R], int j, const cx<T> m0, const cx<T> m1, const cx<T> m2,
                                          const cx<T> m3) {
@@ -472,6 +511,22 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 if (o.c == 15) pass_dense_reg_at<T, KH, R, E, 15>(v, o.a, m0, m1, m2, m3);
                 else if (o.c == 1) pass_dense_reg_at<T, KH, R, E, 1>(v, o.a, m0, m1, m2, m3);
                 else pass_dense_reg_at<T, KH, R, E, 0>(v, o.a, m0, m1, m2, m3);
+                break; }
+            case PO_DENSE_REG_C: {
+                if constexpr (sizeof(T) == 8) {
+                    const cx<T> m0 = c.c0, m1 = c.c1, m2 = c.c2, m3 = c.c3, ph = rec.coef(kPassFoldPhase);
+                    const cx<T> m1p = {m1.re * ph.re - m1.im * ph.im, m1.re * ph.im + m1.im * ph.re};
+                    const cx<T> m3p = {m3.re * ph.re - m3.im * ph.im, m3.re * ph.im + m3.im * ph.re};
+                    if (o.active == 0) {   // control in a register slot
+                        if (o.c == 1) pass_dense_reg_ctl_at<T, KH, R, E, 1>(v, o.a, o.b, m0, m1, m2, m3, m1p, m3p);
+                        else pass_dense_reg_ctl_at<T, KH, R, E, 0>(v, o.a, o.b, m0, m1, m2, m3, m1p, m3p);
+                    } else {               // lane bit (per lane) or tile bit (per wave)
+                        const bool on = o.active == 1 ? ((base >> o.b) & 1) != 0 : ((tb >> o.b) & 1) != 0;
+                        const cx<T> m1s = csel(on, m1p, m1), m3s = csel(on, m3p, m3);
+                        if (o.c == 1) pass_dense_reg_at<T, KH, R, E, 1>(v, o.a, m0, m1s, m2, m3s);
+                        else pass_dense_reg_at<T, KH, R, E, 0>(v, o.a, m0, m1s, m2, m3s);
+                    }
+                }
                 break; }
             case PO_DENSE_LANE: {
                 const bool hi = (lane >> o.a) & 1;

@@ -183,10 +183,52 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
         std::sort(regs.begin(), regs.end());
         l.merged = true;
         l.op_begin = (int)ops.size();
+        std::vector<LoweredGate> lg;
+        lg.reserve(st.gates.size());
         for (int gi : st.gates) {
-            LoweredGate g = low[gi];
-            sq::relabel(g, vpos);
-            encode_pass_op<T>(g, regs, ops);
+            lg.push_back(low[gi]);
+            sq::relabel(lg.back(), vpos);
+        }
+        // complex<double>: a controlled phase (diagonal, only |11> not 1) whose
+        // next gate on either qubit is a dense gate on a register slot goes
+        // into that gate's op, with the other qubit as the control
+        std::vector<int> fold_into(lg.size(), -1), folded_from(lg.size(), -1);
+        if constexpr (sizeof(T) == 8) {
+            auto in_regs = [&](int t) { return std::find(regs.begin(), regs.end(), t) != regs.end(); };
+            for (size_t i = 0; i < lg.size(); i++) {
+                const LoweredGate &d = lg[i];
+                if (!(d.cls == OpClass::Diag && d.k == 2 && d.diag_active == 8u)) continue;
+                for (size_t j = i + 1; j < lg.size(); j++) {
+                    const LoweredGate &u = lg[j];
+                    bool touches = false;
+                    for (int x = 0; x < u.k; x++) touches = touches || u.t[x] == d.t[0] || u.t[x] == d.t[1];
+                    if (!touches) continue;
+                    if (u.cls == OpClass::Dense && u.k == 1 && folded_from[j] < 0 && in_regs(u.t[0])) {
+                        fold_into[i] = (int)j;
+                        folded_from[j] = (int)i;
+                    }
+                    break;
+                }
+            }
+        }
+        for (size_t i = 0; i < lg.size(); i++) {
+            const int gi = st.gates[i];
+            if (fold_into[i] >= 0) {
+                l.flops_per_amp += 6.0;   // applied inside the dense gate's op
+                continue;
+            }
+            encode_pass_op<T>(lg[i], regs, ops);
+            if constexpr (sizeof(T) == 8) if (folded_from[i] >= 0) {
+                const LoweredGate &d = lg[folded_from[i]];
+                PassOp<T> &o = ops.back();
+                const int ctl = lg[i].t[0] == d.t[0] ? d.t[1] : d.t[0];
+                const auto it = std::find(regs.begin(), regs.end(), ctl);
+                o.kind = PO_DENSE_REG_C;
+                if (it != regs.end()) { o.active = 0; o.b = (int)(it - regs.begin()); }
+                else { o.active = ctl < S ? 1 : 2; o.b = ctl; }
+                o.c = o.coef[0].im == (T)0 ? 1 : 0;
+                o.coef[kPassFoldPhase] = {(T)d.m[3].real(), (T)d.m[3].imag()};
+            }
             // dense 2x2 row: two complex products and a sum; diagonal: one product
             l.flops_per_amp += low[gi].cls == OpClass::Dense ? 14.0 : 6.0;
         }
