@@ -169,6 +169,31 @@ def register_pass_circuit(n, n_gates, seed, first_dense):
     return "\n".join(lines) + "\n"
 
 
+@pytest.mark.parametrize("kh", [3, 5])
+@pytest.mark.parametrize("n", [12, 14])
+def test_register_pass_folded_phases(emul, tmp_path, oracle, n, kh):
+    # complex128 passes fold a controlled phase into the next dense gate on a
+    # register qubit; the control may sit in a register slot, a lane bit or a
+    # tile bit.  Each CP below is followed by a U3-form gate on one of its
+    # qubits (dense targets >= 6 only: the emulation has no lane exchange).
+    import random as _random
+    from helpers import gate_line
+    rng = _random.Random(7 * n + kh)
+    nrng = np.random.default_rng(n)
+    lines = []
+    for _ in range(120):
+        a = rng.randrange(6, n)
+        b = rng.choice([rng.randrange(0, 6), rng.randrange(6, n)])
+        if a == b:
+            continue
+        lines.append(f"CP {min(a, b)} {max(a, b)} {rng.uniform(0, 6.28)!r}")
+        lines.append(gate_line(rng.choice(["U3", "U1", "H", "RY"]), [a], rng, nrng))
+    text = "\n".join(lines) + "\n"
+    got = run_case(emul, tmp_path, "plain", n, 4, "c128", text, seed=n, pass_qbits=kh)
+    want = oracle.run(sq.gen_state(n, n), n, text)
+    assert np.max(np.abs(got - want)) <= 1e-12
+
+
 @pytest.mark.parametrize("moves", [1, 3])
 @pytest.mark.parametrize("kh", [2, 5])
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
