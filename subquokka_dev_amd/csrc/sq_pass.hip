@@ -242,7 +242,7 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
     if (!ops.empty()) {
         if (hipStreamSynchronize(e->stream) != hipSuccess) return e->fail(SQ_ERR_HIP, "stream sync");
         if (need > e->passprog_cap) {
-            if (e->d_passprog) hipFree(e->d_passprog);
+            if (e->d_passprog) (void)hipFree(e->d_passprog);
             e->d_passprog = nullptr;
             e->passprog_cap = 0;
             hipError_t he = hipMalloc(&e->d_passprog, need * 2);
