@@ -475,7 +475,7 @@ def main():
            "config": {"workload": f"random_layers_{n}q_{layers}l_seed{SEED}_plain_c128", "n_qubits": n,
                       "layers": layers, "gates": ng, "state": "complex128 in HBM",
                       "mode": "plain gate list (MEM_Runner::run(vector<Gate*>&) form) as register passes, "
-                              "5 register qubits + 6 lane qubits per pass"},
+                              "5 register qubits + 6 lane qubits per pass (3 lane positions chosen per pass)"},
            "hbm_gbs": round(hbm_gbs, 1), "roofline": roofline, "kernels_c128": kernels}
     if out_extra.get("hbm_copy_gbs"):
         out["hbm_copy_ceiling"] = {"gbs": round(out_extra["hbm_copy_gbs"], 1),
