@@ -43,6 +43,8 @@ enum PassOpKind : int32_t {
                          //   (coef[4]).  b = the control's position; active = 0: register
                          //   slot b, 1: lane bit b - VB, 2: tile bit b.  c as PO_DENSE_REG
                          //   (1: m0 real)
+    PO_DENSE_LANE_C = 7, // complex<double> only: the same fold into a dense gate on lane bit a;
+                         //   b, active and coef[4] as PO_DENSE_REG_C
 };
 
 // 32-bit fields only: the op loop reads records with scalar (SMEM) loads,
@@ -409,6 +411,46 @@ __device__ inline void pass_dense_lane(Pack<T> (&v)[R], bool hi, const Coef4<T> 
     }
 }
 
+// PO_DENSE_LANE_C with the control in register slot sj: k0 / k1 are the
+// coefficients for control 0 / 1.  Lane bits 0-3: one uniform branch per
+// register.  Lane bits 4-5: after the swap, a pair (r, r | 1) shares its
+// control unless sj = 0, whose bit the swap has moved into lane bit MASK.
+template <typename T, int MASK, int R, int E>
+__device__ inline void pass_dense_lane_ctl(Pack<T> (&v)[R], bool hi, int sj, const Coef4<T> k0, const Coef4<T> k1,
+                                           bool lane_bit_set) {
+    if constexpr (MASK <= 8) {
+        const cx<T> ms0 = csel(hi, k0.c3, k0.c0), mp0 = csel(hi, k0.c2, k0.c1);
+        const cx<T> ms1 = csel(hi, k1.c3, k1.c0), mp1 = csel(hi, k1.c2, k1.c1);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const cx<T> p = {lane_xor<MASK>(v[r].e[e].re), lane_xor<MASK>(v[r].e[e].im)};
+                if ((r >> sj) & 1) ip_lane(v[r].e[e], p, ms1, mp1);
+                else ip_lane(v[r].e[e], p, ms0, mp0);
+            }
+        }
+    } else {
+        static_assert(R >= 2, "the swap needs a register bit");
+        // sj = 0: the control is the lane's bit MASK once swapped
+        const Coef4<T> kl = {k0.c0, csel(lane_bit_set, k1.c1, k0.c1), k0.c2, csel(lane_bit_set, k1.c3, k0.c3)};
+#pragma unroll
+        for (int r = 0; r < R; r += 2) {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                cx<T> &x = v[r].e[e], &y = v[r + 1].e[e];
+                permlane_swap<MASK>(x.re, y.re);
+                permlane_swap<MASK>(x.im, y.im);
+                if (sj == 0) ip_pair(x, y, kl.c0, kl.c1, kl.c2, kl.c3);
+                else if ((r >> sj) & 1) ip_pair(x, y, k1.c0, k1.c1, k1.c2, k1.c3);
+                else ip_pair(x, y, k0.c0, k0.c1, k0.c2, k0.c3);
+                permlane_swap<MASK>(x.re, y.re);
+                permlane_swap<MASK>(x.im, y.im);
+            }
+        }
+    }
+}
+
 // Diagonal with a bit in register slot J: ph0 / ph1 are the lane's factors
 // for register bit 0 / 1 (per vector element).  A half whose factors are all
 // 1 (act0 / act1 false, wave-uniform) is skipped.
@@ -538,6 +580,39 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                 case 4: pass_dense_lane<T, 16, R, E>(v, hi, c); break;
                 case 5: pass_dense_lane<T, 32, R, E>(v, hi, c); break;
                 default: break;
+                }
+                break; }
+            case PO_DENSE_LANE_C: {
+                if constexpr (sizeof(T) == 8) {
+                    const bool hi = (lane >> o.a) & 1;
+                    const cx<T> ph = rec.coef(kPassFoldPhase);
+                    const cx<T> m1p = {c.c1.re * ph.re - c.c1.im * ph.im, c.c1.re * ph.im + c.c1.im * ph.re};
+                    const cx<T> m3p = {c.c3.re * ph.re - c.c3.im * ph.im, c.c3.re * ph.im + c.c3.im * ph.re};
+                    const Coef4<T> k1 = {c.c0, m1p, c.c2, m3p};
+                    if (o.active == 0) {   // control in register slot o.b
+                        const bool lb = ((lane >> o.a) & 1) != 0;
+                        switch (o.a) {
+                        case 0: pass_dense_lane_ctl<T, 1, R, E>(v, hi, o.b, c, k1, lb); break;
+                        case 1: pass_dense_lane_ctl<T, 2, R, E>(v, hi, o.b, c, k1, lb); break;
+                        case 2: pass_dense_lane_ctl<T, 4, R, E>(v, hi, o.b, c, k1, lb); break;
+                        case 3: pass_dense_lane_ctl<T, 8, R, E>(v, hi, o.b, c, k1, lb); break;
+                        case 4: pass_dense_lane_ctl<T, 16, R, E>(v, hi, o.b, c, k1, lb); break;
+                        case 5: pass_dense_lane_ctl<T, 32, R, E>(v, hi, o.b, c, k1, lb); break;
+                        default: break;
+                        }
+                    } else {               // lane bit (per lane) or tile bit (per wave)
+                        const bool on = o.active == 1 ? ((base >> o.b) & 1) != 0 : ((tb >> o.b) & 1) != 0;
+                        const Coef4<T> k = {c.c0, csel(on, m1p, c.c1), c.c2, csel(on, m3p, c.c3)};
+                        switch (o.a) {
+                        case 0: pass_dense_lane<T, 1, R, E>(v, hi, k); break;
+                        case 1: pass_dense_lane<T, 2, R, E>(v, hi, k); break;
+                        case 2: pass_dense_lane<T, 4, R, E>(v, hi, k); break;
+                        case 3: pass_dense_lane<T, 8, R, E>(v, hi, k); break;
+                        case 4: pass_dense_lane<T, 16, R, E>(v, hi, k); break;
+                        case 5: pass_dense_lane<T, 32, R, E>(v, hi, k); break;
+                        default: break;
+                        }
+                    }
                 }
                 break; }
             case PO_DENSE_VEC: {

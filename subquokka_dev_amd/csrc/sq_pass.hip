@@ -190,8 +190,8 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
             sq::relabel(lg.back(), vpos);
         }
         // complex<double>: a controlled phase (diagonal, only |11> not 1) whose
-        // next gate on either qubit is a dense gate on a register slot goes
-        // into that gate's op, with the other qubit as the control
+        // next gate on either qubit is a dense gate on a register slot or a
+        // lane bit goes into that gate's op, with the other qubit as the control
         std::vector<int> fold_into(lg.size(), -1), folded_from(lg.size(), -1);
         if constexpr (sizeof(T) == 8) {
             auto in_regs = [&](int t) { return std::find(regs.begin(), regs.end(), t) != regs.end(); };
@@ -203,7 +203,8 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
                     bool touches = false;
                     for (int x = 0; x < u.k; x++) touches = touches || u.t[x] == d.t[0] || u.t[x] == d.t[1];
                     if (!touches) continue;
-                    if (u.cls == OpClass::Dense && u.k == 1 && folded_from[j] < 0 && in_regs(u.t[0])) {
+                    if (u.cls == OpClass::Dense && u.k == 1 && folded_from[j] < 0 &&
+                        (in_regs(u.t[0]) || (u.t[0] >= VB && u.t[0] < S))) {
                         fold_into[i] = (int)j;
                         folded_from[j] = (int)i;
                     }
@@ -223,10 +224,11 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
                 PassOp<T> &o = ops.back();
                 const int ctl = lg[i].t[0] == d.t[0] ? d.t[1] : d.t[0];
                 const auto it = std::find(regs.begin(), regs.end(), ctl);
-                o.kind = PO_DENSE_REG_C;
+                const bool lane_target = o.kind == PO_DENSE_LANE;
+                o.kind = lane_target ? PO_DENSE_LANE_C : PO_DENSE_REG_C;
                 if (it != regs.end()) { o.active = 0; o.b = (int)(it - regs.begin()); }
                 else { o.active = ctl < S ? 1 : 2; o.b = ctl; }
-                o.c = o.coef[0].im == (T)0 ? 1 : 0;
+                o.c = !lane_target && o.coef[0].im == (T)0 ? 1 : 0;
                 o.coef[kPassFoldPhase] = {(T)d.m[3].real(), (T)d.m[3].imag()};
             }
             // dense 2x2 row: two complex products and a sum; diagonal: one product
