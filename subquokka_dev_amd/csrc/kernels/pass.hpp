@@ -176,11 +176,12 @@ __device__ inline pass_gv4f *row_ptr(cx<T> *s, uint64_t tb, const uint64_t (&hb)
 }
 
 // Cross-lane exchanges without the LDS.
-// Lane bits 0 and 1: one DPP quad_perm per dword.  Lane bits 2 and 3: two DPP
-// moves that both read the source -- row_shl by 4 (8) written only to the
-// 4-lane banks whose lane bit is 0, then row_shr written to the others -- so
-// no DPP reads a register a VALU op has just written (that would need wait
-// states).  Lane bits 4 and 5: v_permlane16_swap / v_permlane32_swap, which
+// Lane bits 0 and 1: one DPP quad_perm per dword.  Lane bit 2: two DPP moves
+// that both read the source -- row_shl by 4 written only to the 4-lane banks
+// whose lane bit is 0, then row_shr written to the others -- so no DPP reads a
+// register a VALU op has just written (that would need wait states).  Lane
+// bit 3: one row_ror by 8, since rotating a 16-lane row by half its length is
+// lane ^ 8.  Lane bits 4 and 5: v_permlane16_swap / v_permlane32_swap, which
 // exchange the odd 16-lane rows (upper 32 lanes) of one register with the
 // even rows (lower 32 lanes) of another; applied to a register pair that
 // differs in register bit 0, this swaps lane bit 4 (5) with register bit 0,
@@ -193,8 +194,7 @@ template <int MASK> __device__ inline int lane_xor_dword(int x) {
         const int lo = __builtin_amdgcn_mov_dpp(x, 0x104, 0xF, 0x5, false);      // row_shl:4 -> banks 0, 2
         return __builtin_amdgcn_update_dpp(lo, x, 0x114, 0xF, 0xA, false);       // row_shr:4 -> banks 1, 3
     } else {
-        const int lo = __builtin_amdgcn_mov_dpp(x, 0x108, 0xF, 0x3, false);      // row_shl:8 -> banks 0, 1
-        return __builtin_amdgcn_update_dpp(lo, x, 0x118, 0xF, 0xC, false);       // row_shr:8 -> banks 2, 3
+        return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);              // row_ror:8
     }
 }
 template <int MASK, typename T> __device__ inline T lane_xor(T x) {
