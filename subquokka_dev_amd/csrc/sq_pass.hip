@@ -289,7 +289,12 @@ int run_plain_t(sq_engine *e, const PlainProgram &prog, const std::vector<Lowere
         a.n_ops = l.n_ops;
         a.n_tiles = 1ULL << (n - S - kh);
         for (int j = 0; j < kh; j++) a.hpos[j] = l.hpos[j];
-        for (int k = 0; k < kPassLaneHi; k++) a.lh[k] = l.lh[k];
+        for (int k = 0; k < kPassLaneHi; k++) {
+            // the kernel's lane offset holds slot bits below 28 only
+            if (l.lh[k] < VB + 3 || l.lh[k] >= n || l.lh[k] - VB > 27)
+                return e->fail(SQ_ERR_ARG, "internal: lane position out of range");
+            a.lh[k] = l.lh[k];
+        }
         int slot = e->stat_begin(nm, bytes, l.flops_per_amp * std::ldexp(1.0, n));
         int rc;
         switch (kh) {
