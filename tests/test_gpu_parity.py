@@ -173,6 +173,38 @@ def test_register_passes_with_lane_moves(oracle, dtype, kh):
         a.close()
 
 
+@pytest.mark.parametrize("kh", [3, 5])
+def test_register_passes_fold_controlled_phases(oracle, kh):
+    # complex128 passes fold a controlled phase into the next dense gate on
+    # either of its qubits: targets on register slots and on lane bits 0-5,
+    # controls in register slots, lane bits and tile bits
+    import random as _random
+    from helpers import gate_line
+    n = 16
+    rng = _random.Random(100 + kh)
+    nrng = np.random.default_rng(kh)
+    lines = []
+    for _ in range(160):
+        a, b = rng.sample(range(n), 2)
+        lines.append(f"CP {min(a, b)} {max(a, b)} {rng.uniform(0, 6.28)!r}")
+        lines.append(gate_line(rng.choice(["U3", "U1", "H", "RX"]), [a], rng, nrng))
+    text = "\n".join(lines) + "\n"
+    s0 = random_state(n, 100 + kh)
+    say(f"register passes folded phases n={n} kh={kh}")
+    a_ = sq.Engine(n, dtype="c128", pass_qbits=kh)
+    b_ = sq.Engine(n, dtype="c128", sweep_mode=1)
+    try:
+        a_.set_state(s0)
+        b_.set_state(s0)
+        a_.apply_text(text)
+        b_.apply_text(text)
+        assert a_.max_abs_diff(b_) <= 1e-14
+        check(a_.get_state(np.complex128), oracle.run(s0, n, text), "c128", f"folded phases kh={kh}")
+    finally:
+        a_.close()
+        b_.close()
+
+
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("n", [10, 11, 12])
 def test_register_passes_smallest_states(oracle, dtype, n):
