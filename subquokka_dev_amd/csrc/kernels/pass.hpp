@@ -4,9 +4,10 @@
 // the whole state once per gate.  A register pass applies a run of gates to
 // the state in one read and one write: each wave holds a "wave tile" of
 // 2^KH x 64 16-byte vectors in registers -- index bits [0, VB) inside the
-// vector, bits [VB, VB+6) across the lanes and KH chosen qubits hpos[] as the
-// register index -- and applies every gate of the pass in order before
-// storing the tile back.
+// vector, bits [VB, VB+6) across the lanes (the top three of them may stand
+// for other qubits; PassArgs::lh) and KH chosen qubits hpos[] as the register
+// index -- and applies every gate of the pass in order before storing the
+// tile back.
 //
 // Gates a pass can hold:
 //   * dense one-qubit gates on a vector, lane or register bit (H X Y U1 RX RY);
@@ -14,8 +15,10 @@
 //     phase depends only on the amplitude's global index, which every lane
 //     knows.
 // Each gate is its own 2x2 (or diagonal) step in FP arithmetic of the state's
-// type -- nothing is composed -- so a pass agrees with the gates applied one
-// launch at a time up to the rounding of individual products.  Per-op work
+// type -- nothing is composed except a controlled phase folded into the next
+// dense gate on one of its qubits (its phase scales two coefficients) -- so a
+// pass agrees with the gates applied one launch at a time up to the rounding
+// of individual products.  Per-op work
 // is shaped so that nothing is selected per amplitude: lane-gate
 // coefficients are chosen once per lane, shuffle masks are constants, and a
 // diagonal factor is chosen per lane, per register at compile time, or once

@@ -3,8 +3,12 @@
 //
 // The planner (plan_register_passes) cuts the gate list into steps: a merged
 // step is a run of one-qubit dense and diagonal gates whose dense targets lie
-// in the lane bits or in at most pass_qbits chosen register qubits; it costs
-// one read and one write of the state (kernels/pass.hpp).  Every other gate
+// in the resident qubits -- lane bits 0-2 plus up to pass_qbits + 3 chosen
+// qubits, pass_qbits of them in registers and three in the top lane
+// positions -- and costs one read and one write of the state
+// (kernels/pass.hpp).  The encoder writes each step's ops in the pass's
+// virtual positions and folds a controlled phase into the dense gate that
+// follows it on either qubit (complex<double>).  Every other gate
 // (two-qubit dense, permutations) runs alone with its own kernel, and so does
 // a merged step that would cost no more than one pass done gate by gate.
 // Per-qubit gate order is kept, so the schedule computes the same map as the
