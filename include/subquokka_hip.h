@@ -180,9 +180,11 @@ int sq_sample(sq_engine *e, uint64_t seed, uint64_t shots, uint64_t *out);
  * Hardware-efficient ansatz: layers x (RY(theta) on every qubit, RZ(theta) on
  * every qubit, CNOT ring i -> (i+1) mod N); theta has 2*N*layers entries in
  * gate order.  Energy E = <psi| sum_i Z_i Z_{i+1} |psi> from |0...0>.
- * checkpoint_every: layers between re-anchoring checkpoints (0 = none). */
-int sq_hea_energy(sq_engine *e, int32_t layers, const double *theta, double *energy);
-int sq_adjoint_grad_hea(sq_engine *e, int32_t layers, const double *theta,
+ * n_theta must equal 2*N*layers (SQ_ERR_ARG otherwise); grad receives
+ * n_theta values.  checkpoint_every: layers between re-anchoring checkpoints
+ * (0 = none). */
+int sq_hea_energy(sq_engine *e, int32_t layers, const double *theta, uint64_t n_theta, double *energy);
+int sq_adjoint_grad_hea(sq_engine *e, int32_t layers, const double *theta, uint64_t n_theta,
                         int32_t checkpoint_every, double *energy, double *grad);
 
 /* ---- per-kernel timing (HIP events on the engine's stream) ---- */

@@ -168,17 +168,21 @@ int gradient(sq_engine *e, int layers, const double *theta, int ckpt_every, doub
 
 }  // namespace
 
-extern "C" int sq_hea_energy(sq_engine *e, int32_t layers, const double *theta, double *energy) {
+extern "C" int sq_hea_energy(sq_engine *e, int32_t layers, const double *theta, uint64_t n_theta, double *energy) {
     if (!e || layers < 0 || (layers && !theta)) return SQ_ERR_ARG;
+    if (n_theta != (uint64_t)2 * e->n * layers)
+        return e->fail(SQ_ERR_ARG, "theta must hold 2 * total_qbit * layers angles");
     if (e->out_of_core()) return e->fail(SQ_ERR_UNSUPPORTED, "the ansatz needs an HBM-resident state");
     if (hipSetDevice(e->device) != hipSuccess) return SQ_ERR_HIP;
     return e->dtype == SQ_C128 ? gradient<double>(e, layers, theta, 0, energy, nullptr)
                                : gradient<float>(e, layers, theta, 0, energy, nullptr);
 }
 
-extern "C" int sq_adjoint_grad_hea(sq_engine *e, int32_t layers, const double *theta, int32_t checkpoint_every,
-                                   double *energy, double *grad) {
+extern "C" int sq_adjoint_grad_hea(sq_engine *e, int32_t layers, const double *theta, uint64_t n_theta,
+                                   int32_t checkpoint_every, double *energy, double *grad) {
     if (!e || layers < 0 || (layers && (!theta || !grad)) || checkpoint_every < 0) return SQ_ERR_ARG;
+    if (n_theta != (uint64_t)2 * e->n * layers)
+        return e->fail(SQ_ERR_ARG, "theta must hold 2 * total_qbit * layers angles");
     if (e->n < 2) return e->fail(SQ_ERR_ARG, "the ansatz needs at least two qubits");
     if (e->out_of_core()) return e->fail(SQ_ERR_UNSUPPORTED, "the ansatz needs an HBM-resident state");
     if (hipSetDevice(e->device) != hipSuccess) return SQ_ERR_HIP;

@@ -296,8 +296,9 @@ int create_engine(const sq_config *cfg, const char *const *files, int n_files, b
     e->dtype = cfg->dtype;
     e->device = cfg->device;
     e->sweep_mode = cfg->sweep_mode;
-    // register qubits per pass: 4 for complex<double>; 3 for complex<float>,
-    // whose 4-qubit pass exceeds the register file at two waves per SIMD
+    // register qubits per pass: five for both state types (2^5 rows of 16-byte
+    // vectors per wave fit the register file at three waves per SIMD without
+    // spills; fewer, longer passes were measured faster than four)
     e->pass_qbits = cfg->pass_qbits > 0 ? cfg->pass_qbits : 5;
     if (hipSetDevice(e->device) != hipSuccess) { delete e; return SQ_ERR_HIP; }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { delete e; return SQ_ERR_HIP; }

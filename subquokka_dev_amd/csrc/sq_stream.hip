@@ -102,7 +102,13 @@ int sq_engine::io_fail(const char *what) {
     return fail(SQ_ERR_IO, m);
 }
 
-int sq_engine::stream_high() const { return std::min(kMaxHigh, std::min(res_qbit - 1, n - res_qbit + 2)); }
+// Free slots of a pass above its fixed low run.  At least four, so that a
+// VSWAP_2_2 whose four qubits all lie above the run still fits one pass
+// (resident_qbit >= 8 keeps a fixed run of at least one qubit); beyond that,
+// up to n - W + 2 (fewer, longer runs when few qubits lie outside the chunk).
+int sq_engine::stream_high() const {
+    return std::min(kMaxHigh, std::min(res_qbit - 1, std::max(4, n - res_qbit + 2)));
+}
 
 int sq_engine::stream_create() {
     const size_t state = state_bytes();

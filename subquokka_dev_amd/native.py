@@ -78,10 +78,11 @@ _SIGS = {
     "sq_expect_zz_ring": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
     "sq_inner": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "sq_max_abs_diff": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_double)]),
-    "sq_hea_energy": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+    "sq_hea_energy": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_uint64,
                                      ctypes.POINTER(ctypes.c_double)]),
-    "sq_adjoint_grad_hea": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_int32,
-                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "sq_adjoint_grad_hea": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_uint64,
+                                           ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double)]),
     "sq_sample": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _P]),
     "sq_stats_enable": (ctypes.c_int, [_P, ctypes.c_int32]),
     "sq_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(SqKstat), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
@@ -143,7 +144,9 @@ def gen_circuit(kind, n, layers=0, seed=0, theta=None):
     k = {"qft": 0, "random": 1, "hea": 2}[kind]
     th = None
     if theta is not None:
-        arr = np.ascontiguousarray(theta, dtype=np.float64)
+        arr = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+        if kind != "hea" or arr.size != 2 * n * layers:
+            raise ValueError("theta is for the 'hea' circuit and holds 2 * n * layers angles")
         th = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     need = ctypes.c_uint64(0)
     lib().sq_gen_circuit(k, n, layers, seed, th, None, 0, ctypes.byref(need))
@@ -339,18 +342,24 @@ class Engine:
                                            alg_bytes=arr[i].alg_bytes, alg_flops=arr[i].alg_flops)
                 for i in range(n.value)}
 
+    def _hea_theta(self, layers, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+        if th.size != 2 * self.n * layers:
+            raise ValueError(f"theta holds {th.size} angles; the ansatz has 2 * {self.n} * {layers}")
+        return th
+
     def hea_energy(self, layers, theta):
-        th = np.ascontiguousarray(theta, dtype=np.float64)
+        th = self._hea_theta(layers, theta)
         v = ctypes.c_double()
-        self._c(lib().sq_hea_energy(self.h, layers, th.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+        self._c(lib().sq_hea_energy(self.h, layers, th.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), th.size,
                                     ctypes.byref(v)))
         return v.value
 
     def adjoint_grad_hea(self, layers, theta, checkpoint_every=0):
-        th = np.ascontiguousarray(theta, dtype=np.float64)
+        th = self._hea_theta(layers, theta)
         g = np.empty_like(th)
         v = ctypes.c_double()
         self._c(lib().sq_adjoint_grad_hea(self.h, layers, th.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                          checkpoint_every, ctypes.byref(v),
+                                          th.size, checkpoint_every, ctypes.byref(v),
                                           g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return v.value, g

@@ -21,6 +21,18 @@ def oracle():
     return Oracle()
 
 
+@pytest.fixture
+def odirect_state_dir(tmp_path):
+    """(directory accepting O_DIRECT or None, refusals): see helpers.odirect_dir.
+    A folder made outside tmp_path is removed after the test."""
+    import shutil
+    from helpers import odirect_dir
+    d, seen = odirect_dir(tmp_path)
+    yield d, seen
+    if d is not None and d != str(tmp_path):
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def _gpu_present():
     # The kernel driver node; checking it does not initialise the GPU.
     return os.path.exists("/dev/kfd")

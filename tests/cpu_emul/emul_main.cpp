@@ -1,7 +1,9 @@
 // Driver of the CPU emulation build: runs one circuit through the C ABI with
 // the engine compiled for the host (see hip/hip_runtime.h in this folder).
-//   emul <plain|sub|fused> <n> <chunk> <c128|c64> <circuit file>
-// (fused: subcircuit-format file whose blocks act on at most five qubits)
+//   emul <plain|sub|fused|ooc|oocsub|adjoint> <n> <chunk> <c128|c64> <circuit file> [out]
+// (fused: subcircuit-format file whose blocks act on at most five qubits;
+// ooc / oocsub: the out-of-core runner with resident_qbit = chunk, on a plain
+// gate list / a subcircuit file)
 // Exit code 0 means every launch stayed in bounds (AddressSanitizer aborts
 // the process on the first bad access).
 #include <cstdio>
@@ -20,8 +22,8 @@ namespace sqk { alignas(16) unsigned char lds_raw[1 << 16]; }
 int main(int argc, char **argv) {
     if (argc != 6 && argc != 7) { std::fprintf(stderr, "usage: emul plain|sub|fused n chunk c128|c64 file [out]\n"); return 2; }
     const std::string mode = argv[1];
-    const bool sub = mode == "sub" || mode == "fused";
-    const bool ooc = mode == "ooc";   // out-of-core: chunk argument = resident_qbit
+    const bool sub = mode == "sub" || mode == "fused" || mode == "oocsub";
+    const bool ooc = mode == "ooc" || mode == "oocsub";   // out-of-core: chunk argument = resident_qbit
     const int n = std::atoi(argv[2]), chunk = std::atoi(argv[3]);
     const int dtype = std::string(argv[4]) == "c64" ? SQ_C64 : SQ_C128;
     std::ifstream f(argv[5]);
@@ -43,7 +45,8 @@ int main(int argc, char **argv) {
     if (const char *sm = std::getenv("EMUL_SWEEP_MODE")) cfg.sweep_mode = std::atoi(sm);
     sq_engine *e = nullptr;
     const char *seed_env = std::getenv("EMUL_INIT_SEED");
-    // out-of-core host tier: EMUL_HOST_TIER = pageable | files (files in EMUL_FILES_DIR)
+    // out-of-core host tier: EMUL_HOST_TIER = pageable | files (files in
+    // EMUL_FILES_DIR, opened O_DIRECT when EMUL_DIRECT_IO=1)
     const char *tier = std::getenv("EMUL_HOST_TIER");
     std::vector<std::string> paths;
     if (ooc && tier && std::string(tier) == "pageable") cfg.host_tier = SQ_HOST_PAGEABLE;
@@ -52,11 +55,18 @@ int main(int argc, char **argv) {
         for (int i = 0; i < 4; i++) paths.push_back(std::string(dir ? dir : ".") + "/state" + std::to_string(i) + ".bin");
         std::vector<const char *> cp;
         for (auto &p : paths) cp.push_back(p.c_str());
-        rc = sq_create_file_state(&cfg, cp.data(), 4, 0, 0, &e);
+        const char *dio = std::getenv("EMUL_DIRECT_IO");
+        rc = sq_create_file_state(&cfg, cp.data(), 4, dio && std::atoi(dio) ? 1 : 0, 0, &e);
     } else {
         rc = sq_create(&cfg, &e);
     }
     if (rc) { std::fprintf(stderr, "create rc=%d\n", rc); return 4; }
+    if (const char *want = std::getenv("EMUL_EXPECT_TIER")) {   // the tier the engine reports
+        uint64_t passes = 0;
+        double bytes = 0;
+        int32_t t = -1;
+        if (sq_stream_info(e, &passes, &bytes, &t) || t != std::atoi(want)) { std::fprintf(stderr, "tier %d\n", t); return 14; }
+    }
     // Cross-lane exchanges are not modelled here (one lane runs at a time), so
     // register passes keep the lane qubits in place unless EMUL_LANE_MOVES asks
     // otherwise (then only the addressing is checked, not the values).
@@ -67,9 +77,10 @@ int main(int argc, char **argv) {
         std::vector<double> theta((size_t)2 * n * layers), grad(theta.size());
         for (size_t i = 0; i < theta.size(); i++) theta[i] = 0.1 + 0.37 * (double)i;
         double energy = 0;
-        rc = sq_adjoint_grad_hea(e, layers, theta.data(), 1, &energy, grad.data());
+        rc = sq_adjoint_grad_hea(e, layers, theta.data(), theta.size(), 1, &energy, grad.data());
         if (rc) { std::fprintf(stderr, "adjoint rc=%d %s\n", rc, sq_last_error(e)); return 10; }
-        rc = sq_hea_energy(e, layers, theta.data(), &energy);
+        rc = sq_hea_energy(e, layers, theta.data(), theta.size(), &energy);
+        if (!rc && sq_hea_energy(e, layers, theta.data(), theta.size() - 1, &energy) != SQ_ERR_ARG) return 12;
         if (rc) return 11;
         sq_destroy(e);
         std::printf("ok adjoint params=%zu\n", theta.size());
@@ -97,7 +108,8 @@ int main(int argc, char **argv) {
         if ((rc = sq_sample(e, 5, shots.size(), shots.data()))) { std::fprintf(stderr, "sample rc=%d %s\n", rc, sq_last_error(e)); return 12; }
         for (uint64_t v : shots) if (v >> n) { std::fprintf(stderr, "shot out of range\n"); return 13; }
     }
+    const int direct = e->file_direct ? 1 : 0;
     sq_destroy(e);
-    std::printf("ok gates=%llu blocks=%llu\n", (unsigned long long)ng, (unsigned long long)nb);
+    std::printf("ok gates=%llu blocks=%llu direct_io=%d\n", (unsigned long long)ng, (unsigned long long)nb, direct);
     return 0;
 }

@@ -328,3 +328,91 @@ def sample_oracle(state, seed, shots):
             j = jlast
         out.append((b * spb + m) * 16 + j)
     return np.array(out, dtype=np.uint64)
+
+
+def fs_type(path):
+    """Filesystem type of the mount that holds path (from /proc/mounts)."""
+    path = os.path.realpath(path)
+    best, kind = "", "?"
+    try:
+        with open("/proc/mounts") as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 3 and (path == parts[1] or path.startswith(parts[1].rstrip("/") + "/")):
+                    if len(parts[1]) >= len(best):
+                        best, kind = parts[1], parts[2]
+    except OSError:
+        pass
+    return kind
+
+
+def odirect_ok(directory):
+    """True if a 4 KiB O_DIRECT write and read succeed in directory."""
+    import mmap
+    probe = os.path.join(directory, f".odirect_probe_{os.getpid()}")
+    try:
+        fd = os.open(probe, os.O_RDWR | os.O_CREAT | os.O_DIRECT, 0o644)
+    except OSError:
+        return False
+    try:
+        buf = mmap.mmap(-1, 4096)   # page-aligned
+        buf.write(b"\x5a" * 4096)
+        if os.pwrite(fd, buf, 0) != 4096:
+            return False
+        back = mmap.mmap(-1, 4096)
+        return os.preadv(fd, [back], 0) == 4096 and back[:] == buf[:]
+    except OSError:
+        return False
+    finally:
+        os.close(fd)
+        os.unlink(probe)
+
+
+def odirect_dir(tmp_path):
+    """A directory that accepts O_DIRECT: tmp_path itself, else a new temporary
+    folder in $SQ_ODIRECT_DIR, the repository root, /var/tmp or /dev/shm (the
+    caller removes it).  Returns (path or None, what each candidate's
+    filesystem did)."""
+    cands = [str(tmp_path)]
+    if os.environ.get("SQ_ODIRECT_DIR"):
+        cands.append(os.environ["SQ_ODIRECT_DIR"])
+    cands += [ROOT, "/var/tmp", "/dev/shm"]
+    seen = []
+    for c in cands:
+        if not os.path.isdir(c):
+            seen.append(f"{c} (absent)")
+            continue
+        if odirect_ok(c):
+            if c == str(tmp_path):
+                return c, seen
+            import tempfile
+            return tempfile.mkdtemp(prefix=".sq_odirect_", dir=c), seen
+        seen.append(f"{c} ({fs_type(c)}: O_DIRECT refused)")
+    return None, seen
+
+
+def high_qubit_subcircuit(n, chunk, seed, n_blocks=8):
+    """A subcircuit file whose gates and VSWAPs act on qubits >= 7 only, so the
+    emulation (which does not model cross-lane exchanges) computes every value
+    exactly: gate blocks on qubits [7, chunk) between VSWAP_2_2 / VSWAP_1_1
+    passes that swap chunk qubits with qubits >= chunk."""
+    rng = random.Random(seed)
+    nrng = np.random.default_rng(seed)
+    blocks = []
+    for b in range(n_blocks):
+        lines = []
+        for _ in range(12):
+            k = rng.choice(["H", "U3", "RX", "RZ", "P", "CP", "RZZ", "U2", "SWAP"])
+            if k in ("CP", "RZZ", "U2", "SWAP"):
+                qs = sorted(rng.sample(range(7, chunk), 2))
+            else:
+                qs = [rng.randrange(7, chunk)]
+            lines.append(gate_line(k, qs, rng, nrng))
+        blocks.append(lines)
+        if b % 2 == 0:
+            lo = sorted(rng.sample(range(7, chunk), 2))
+            hi = sorted(rng.sample(range(chunk, n), 2))
+            blocks.append([f"VSWAP_2_2 {lo[0]} {lo[1]} {hi[0]} {hi[1]}"])
+        else:
+            blocks.append([f"VSWAP_1_1 {rng.randrange(7, chunk)} {rng.randrange(chunk, n)}"])
+    return "".join(f"{len(bl)}\n" + "\n".join(bl) + "\n" for bl in blocks)

@@ -82,22 +82,58 @@ def test_cli_plain_matches_oracle(tmp_path, oracle, runner):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("runner", ["DirectIO", "IO"])
-def test_cli_state_files_hold_the_result(tmp_path, oracle, runner):
+def test_cli_state_files_hold_the_result(tmp_path, oracle, odirect_state_dir, runner):
     # runner_type DirectIO / IO with state_paths: the state lives in 2^file_qbit
     # files (setupFiles, simulator.cpp:102-129) and holds the result afterwards
     n = 16
     text = random_circuit(n, 120, seed=5)
     open(tmp_path / "c.txt", "w").write(text)
-    paths = ",".join(f"./state/path{i}" for i in range(4))
+    sdir = cli_state_dir(tmp_path, runner, odirect_state_dir)
+    paths = ",".join(os.path.join(sdir, f"path{i}") for i in range(4))
     write_ini(tmp_path / "a.ini", total_qbit=n, file_qbit=2, chunk_qbit=5, runner_type=runner,
               is_subcircuit=0, init_seed=9, resident_qbit=14, state_paths=paths)
     r = run(["-i", "a.ini", "-c", "c.txt"], tmp_path)
-    if r.returncode != 0 and runner == "DirectIO" and ("create failed" in r.stderr or "(O_DIRECT)" in r.stderr):
-        pytest.skip("O_DIRECT is not supported on this filesystem: " + r.stderr[-200:])
     assert r.returncode == 0, r.stderr
     assert ("DIO Mode" if runner == "DirectIO" else "IO Mode") in r.stdout
-    got = np.concatenate([np.fromfile(tmp_path / "state" / f"path{i}", dtype=np.complex128) for i in range(4)])
+    got = np.concatenate([np.fromfile(os.path.join(sdir, f"path{i}"), dtype=np.complex128) for i in range(4)])
     np.testing.assert_allclose(got, oracle.run(sq.gen_state(n, 9), n, text, chunk=5), rtol=0, atol=1e-12)
+
+
+def cli_state_dir(tmp_path, runner, odirect):
+    """Directory for the state files: for DirectIO the first candidate whose
+    filesystem accepts O_DIRECT (skip, naming what refused, if none)."""
+    if runner != "DirectIO":
+        return str(tmp_path / "state")
+    d, seen = odirect
+    if d is None:
+        pytest.skip("no directory on this host accepts O_DIRECT: " + "; ".join(seen))
+    print(f"O_DIRECT state files in {d}", flush=True)
+    return os.path.join(d, "state")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("runner,files", [("DirectIO", True), ("IO", True), ("IO", False)])
+def test_cli_subcircuit_on_the_out_of_core_runner(tmp_path, oracle, odirect_state_dir, runner, files):
+    # runner_type DirectIO / IO at the default is_subcircuit=1 (simulator.cpp:149):
+    # DIO_Runner::run(subcircuits) (DIO_Runner.cpp:399) on a chunk-12 finder plan
+    n = 17
+    text = sq.plan(random_circuit(n, 160, seed=6), n, 0, 12)
+    assert "VSWAP_2_2" in text
+    open(tmp_path / "c.txt", "w").write(text)
+    kv = dict(total_qbit=n, file_qbit=2, chunk_qbit=12, runner_type=runner, init_seed=11, resident_qbit=15,
+              dump_binary="out.bin")
+    if files:
+        sdir = cli_state_dir(tmp_path, runner, odirect_state_dir)
+        kv["state_paths"] = ",".join(os.path.join(sdir, f"path{i}") for i in range(4))
+    write_ini(tmp_path / "a.ini", **kv)
+    r = run(["-i", "a.ini", "-c", "c.txt"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert ("DIO Mode" if runner == "DirectIO" else "IO Mode") in r.stdout
+    want = oracle.run(sq.gen_state(n, 11), n, text, chunk=12, sub=True)
+    np.testing.assert_allclose(dumped(tmp_path / "out.bin", n), want, rtol=0, atol=1e-12)
+    if files:
+        got = np.concatenate([np.fromfile(os.path.join(sdir, f"path{i}"), dtype=np.complex128) for i in range(4)])
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
 
 
 @pytest.mark.gpu

@@ -225,3 +225,82 @@ def test_register_pass_numerics(emul, tmp_path, oracle, dtype, n, first_dense, k
     want = oracle.run(sq.gen_state(n, n), n, text)
     tol = 1e-12 if dtype == "c128" else 2e-6
     assert np.max(np.abs(got - want)) <= tol
+
+
+def test_out_of_core_direct_io_matches_pinned(emul, tmp_path, odirect_state_dir):
+    # runner_type DirectIO: the state files opened O_DIRECT (create_file,
+    # simulator.cpp:71-86), every transfer a 4 KiB-aligned run of the staging
+    # buffer (DIO_Runner.cpp:155-164); the bits must equal the pinned tier's
+    d, seen = odirect_state_dir
+    if d is None:
+        pytest.skip("no directory here accepts O_DIRECT: " + "; ".join(seen))
+    n, w = 16, 14   # runs of 2^10 amplitudes: 16 / 8 KiB, 4 KiB-aligned for both dtypes
+    text = sq.gen_circuit("random", n, 2, seed=4) + random_circuit(n, 60, seed=44)
+    for dtype in ("c128", "c64"):
+        base = run_case(emul, tmp_path, "ooc", n, w, dtype, text, seed=3)
+        os.environ["EMUL_DIRECT_IO"] = "1"
+        try:
+            f = tmp_path / "c.txt"
+            f.write_text(text)
+            out = tmp_path / "direct.bin"
+            env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", EMUL_INIT_SEED="3",
+                       EMUL_HOST_TIER="files", EMUL_FILES_DIR=d, EMUL_EXPECT_TIER="3")
+            r = subprocess.run([emul, "ooc", str(n), str(w), dtype, str(f), str(out)], capture_output=True,
+                               text=True, timeout=900, env=env)
+        finally:
+            os.environ.pop("EMUL_DIRECT_IO", None)
+        assert r.returncode == 0, (r.stdout + r.stderr)[-3000:]
+        assert "direct_io=1" in r.stdout
+        raw = np.fromfile(out, dtype=np.float64)
+        assert np.array_equal(raw[0::2] + 1j * raw[1::2], base)
+        files = [os.path.join(d, f"state{i}.bin") for i in range(4)]
+        # the files hold the final state in the reference's layout
+        dt = np.complex128 if dtype == "c128" else np.complex64
+        on_disk = np.concatenate([np.fromfile(p, dtype=dt) for p in files]).astype(np.complex128)
+        assert np.array_equal(on_disk, base)
+
+
+def test_out_of_core_direct_io_rejects_unaligned_runs(emul, tmp_path):
+    # runs of 2^(W - h) amplitudes below 4 KiB cannot be O_DIRECT transfers
+    n, w = 12, 9
+    f = tmp_path / "c.txt"
+    f.write_text("H 0\n")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", EMUL_HOST_TIER="files",
+               EMUL_FILES_DIR=str(tmp_path), EMUL_DIRECT_IO="1")
+    r = subprocess.run([emul, "ooc", str(n), str(w), "c128", str(f)], capture_output=True, text=True, timeout=300,
+                       env=env)
+    assert r.returncode == 4 and "4 KiB" in r.stderr, (r.stdout + r.stderr)[-2000:]
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n,w", [(16, 13), (13, 12)])
+def test_out_of_core_subcircuit_matches_oracle(emul, tmp_path, oracle, dtype, n, w):
+    # DIO_Runner::run(subcircuits) (DIO_Runner.cpp:399), the DIO runner's
+    # default mode (is_subcircuit = 1, simulator.cpp:149): gate blocks and
+    # VSWAP passes streamed through HBM, every host tier the same bits, all of
+    # them equal to the oracle's subcircuit sweep.  n = w + 1 puts the VSWAP_2_2
+    # qubits above the pass's fixed low run (four free slots are needed).  Both
+    # shapes keep qubits 0-7 in every pass's fixed run, so the lane bits hold
+    # no gate and the emulation's values are exact.
+    chunk = n - 2
+    from helpers import high_qubit_subcircuit
+    text = high_qubit_subcircuit(n, chunk, seed=n + w)
+    want = oracle.run(sq.gen_state(n, 5), n, text, chunk=chunk, sub=True)
+    base = run_case(emul, tmp_path, "oocsub", n, w, dtype, text, seed=5)
+    tol = 1e-12 if dtype == "c128" else 2e-6
+    assert np.max(np.abs(base - want)) <= tol
+    for tier in ("pageable", "files"):
+        got = run_case(emul, tmp_path, "oocsub", n, w, dtype, text, seed=5, tier=tier)
+        assert np.array_equal(got, base), tier
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_out_of_core_finder_plan_launches(emul, tmp_path, dtype):
+    # a finder-format plan (chunk 10, VSWAP_2_2 / VSWAP_1_1 on any qubits)
+    # through the out-of-core runner: every launch in bounds, tiers agree
+    n, w = 15, 12
+    text = sq.plan(random_circuit(n, 150, seed=9), n, 0, 10)
+    assert "VSWAP_2_2" in text
+    base = run_case(emul, tmp_path, "oocsub", n, w, dtype, text, seed=2)
+    got = run_case(emul, tmp_path, "oocsub", n, w, dtype, text, seed=2, tier="files")
+    assert np.array_equal(got, base)

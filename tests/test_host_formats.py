@@ -132,6 +132,27 @@ def test_gen_hea_structure():
     assert float(lines[0].split()[2]) == theta[0]
 
 
+def test_hea_theta_length_is_checked_before_any_call():
+    # a theta of the wrong length must never reach the C side (which reads
+    # 2 * n * layers angles and writes as many gradients)
+    n, layers = 6, 2
+    with pytest.raises(ValueError):
+        sq.gen_circuit("hea", n, layers, theta=np.zeros(2 * n * layers - 1))
+    with pytest.raises(ValueError):
+        sq.gen_circuit("random", n, layers, theta=np.zeros(2 * n * layers))
+    eng = sq.Engine.__new__(sq.Engine)   # no device: the checks come first
+    eng.h, eng.n, eng.dtype = None, n, "c128"
+    for bad in (np.zeros(2 * n * layers - 1), np.zeros(2 * n * layers + 1), np.zeros((2, n * layers + 1))):
+        with pytest.raises(ValueError):
+            eng.hea_energy(layers, bad)
+        with pytest.raises(ValueError):
+            eng.adjoint_grad_hea(layers, bad)
+    torch = pytest.importorskip("torch")
+    from subquokka_dev_amd.autograd import hea_energy
+    with pytest.raises(ValueError):
+        hea_energy(torch.zeros(2 * n * layers - 2, dtype=torch.float64), eng, layers)
+
+
 def test_gen_state_normalised_and_seeded():
     s1 = sq.gen_state(12, 1)
     s2 = sq.gen_state(12, 1)
