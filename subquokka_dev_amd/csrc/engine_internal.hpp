@@ -88,8 +88,10 @@ struct sq_engine {
     int stream_create();
     int stream_open_files(const char *const *paths, int n_files, bool direct, bool keep);
     int stream_high() const;
-    bool host_read(uint64_t off, size_t bytes, void *dst);
-    bool host_write(uint64_t off, size_t bytes, const void *src);
+    // bounce: an ordinary page-aligned buffer of at least `bytes`, through which
+    // O_DIRECT file transfers go (required when file_direct)
+    bool host_read(uint64_t off, size_t bytes, void *dst, char *bounce);
+    bool host_write(uint64_t off, size_t bytes, const void *src, char *bounce);
     bool stage_runs(bool to_stage, uint64_t base_bytes, const std::vector<uint64_t> &run_off_bytes, size_t run_bytes,
                     char *stage);
     template <typename F> int host_chunks(bool write_back, F fn);

@@ -204,35 +204,43 @@ int sq_engine::stream_open_files(const char *const *paths, int n_files, bool dir
 }
 
 // Host-tier byte access at global byte offset off (ranges may cross files).
-bool sq_engine::host_read(uint64_t off, size_t bytes, void *dst) {
+// O_DIRECT transfers DMA straight between the file and the user pages, and
+// the kernel refuses (EFAULT) pinned memory from hipHostMalloc as their
+// target, so with file_direct every transfer goes through `bounce`, an
+// ordinary page-aligned buffer, and a memcpy to or from the staging memory.
+bool sq_engine::host_read(uint64_t off, size_t bytes, void *dst, char *bounce) {
     if (host_tier != SQ_HOST_FILES) {
         std::memcpy(dst, static_cast<char *>(h_state) + off, bytes);
         return true;
     }
+    if (file_direct && !bounce) { io_errno = EINVAL; return false; }
     const uint64_t per_file = state_bytes() / file_fds.size();
     char *d = static_cast<char *>(dst);
     while (bytes) {
         const uint64_t f = off / per_file, fo = off % per_file;
         const size_t take = (size_t)std::min<uint64_t>(bytes, per_file - fo);
         int err = 0;
-        if (!pread_all(file_fds[f], d, take, (off_t)fo, &err)) { io_errno = err; return false; }
+        if (!pread_all(file_fds[f], file_direct ? bounce : d, take, (off_t)fo, &err)) { io_errno = err; return false; }
+        if (file_direct) std::memcpy(d, bounce, take);
         d += take; off += take; bytes -= take;
     }
     return true;
 }
 
-bool sq_engine::host_write(uint64_t off, size_t bytes, const void *src) {
+bool sq_engine::host_write(uint64_t off, size_t bytes, const void *src, char *bounce) {
     if (host_tier != SQ_HOST_FILES) {
         std::memcpy(static_cast<char *>(h_state) + off, src, bytes);
         return true;
     }
+    if (file_direct && !bounce) { io_errno = EINVAL; return false; }
     const uint64_t per_file = state_bytes() / file_fds.size();
     const char *s = static_cast<const char *>(src);
     while (bytes) {
         const uint64_t f = off / per_file, fo = off % per_file;
         const size_t take = (size_t)std::min<uint64_t>(bytes, per_file - fo);
+        if (file_direct) std::memcpy(bounce, s, take);
         int err = 0;
-        if (!pwrite_all(file_fds[f], s, take, (off_t)fo, &err)) { io_errno = err; return false; }
+        if (!pwrite_all(file_fds[f], file_direct ? bounce : s, take, (off_t)fo, &err)) { io_errno = err; return false; }
         s += take; off += take; bytes -= take;
     }
     return true;
@@ -251,15 +259,22 @@ bool sq_engine::stage_runs(bool to_stage, uint64_t base_bytes, const std::vector
     std::vector<std::thread> th;
     for (unsigned k = 0; k < t; k++)
         th.emplace_back([&, k] {
+            // O_DIRECT: one page-aligned bounce buffer per thread (host_read)
+            char *bounce = nullptr;
+            if (host_tier == SQ_HOST_FILES && file_direct) {
+                bounce = static_cast<char *>(std::aligned_alloc(kIoAlign, (piece + kIoAlign - 1) / kIoAlign * kIoAlign));
+                if (!bounce) { io_errno = ENOMEM; ok[k] = 0; return; }
+            }
             for (uint64_t p = total * k / t; p < total * (k + 1) / t; p++) {
                 const uint64_t r = p / per_run, i = p % per_run;
                 const size_t o = (size_t)i * piece;
                 const size_t len = std::min(piece, run_bytes - o);
                 char *sp = stage + r * run_bytes + o;
                 const uint64_t hp = base_bytes + run_off_bytes[r] + o;
-                const bool good = to_stage ? host_read(hp, len, sp) : host_write(hp, len, sp);
+                const bool good = to_stage ? host_read(hp, len, sp, bounce) : host_write(hp, len, sp, bounce);
                 if (!good) ok[k] = 0;
             }
+            std::free(bounce);
         });
     for (auto &x : th) x.join();
     return std::all_of(ok.begin(), ok.end(), [](char c) { return c != 0; });
@@ -310,7 +325,8 @@ int sq_engine::stream_init_zero() {
         char *buf = static_cast<char *>(h_stage[0]);
         std::memset(buf, 0, kIoAlign);
         std::memcpy(buf, one, B);
-        if (!host_write(0, std::min<size_t>(kIoAlign, state_bytes()), buf)) return io_fail("state file write");
+        alignas(kIoAlign) char bounce[kIoAlign];
+        if (!host_write(0, std::min<size_t>(kIoAlign, state_bytes()), buf, bounce)) return io_fail("state file write");
     }
     return SQ_OK;
 }
