@@ -140,6 +140,20 @@ const char *sq_last_error(const sq_engine *e);
 /* ---- the hot path ---- */
 /* Plain gate-by-gate sweep: MEM_Runner::run(vector<Gate*>&) (MEM_Runner.cpp:101-134). */
 int sq_apply_gates(sq_engine *e, const sq_gate *gates, uint64_t n);
+/* sq_apply_gates with no allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same kernels and result
+ * as sq_apply_gates on an HBM-resident engine; the register-pass program goes
+ * into a fixed 4 MiB device array (one per GPU, set up by sq_create, held until
+ * the process ends) instead of a buffer the engine grows: a list whose program
+ * exceeds it (about 32768 merged gates in complex128, 65536 in complex64) fails
+ * with SQ_ERR_ARG before anything runs.  The call still synchronises the
+ * engine's stream before it writes the array, as sq_apply_gates does before it
+ * uploads.  Every engine on a GPU shares that GPU's array, which keeps the last
+ * call's program: calls that overlap on one GPU (from two host threads, or two
+ * engines whose streams run at once) race on it.  SQ_ERR_UNSUPPORTED for the
+ * out-of-core runner. */
+int sq_apply_gates_noalloc(sq_engine *e, const sq_gate *gates, uint64_t n);
 /* Subcircuit sweep: MEM_Runner::run(vector<vector<Gate*>>&) (MEM_Runner.cpp:136-185).
  * block_sizes[i] gates form block i; blocks whose first gate is VSWAP_* are
  * qubit-swap passes, the others gate blocks.  The final state, in the layout the

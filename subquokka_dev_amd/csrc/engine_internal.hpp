@@ -31,6 +31,7 @@ struct PlainProgram {
     int n = 0, kh = 0;
     bool per_gate = false;         // one launch per gate (sweep_mode 1, small states)
     std::vector<PlainLaunch> launches;
+    const void *d_ops = nullptr;   // device copy of the op records the launches slice
 };
 
 struct sq_engine {
@@ -65,6 +66,7 @@ struct sq_engine {
     size_t sample_cap = 0;
     void *d_passprog = nullptr;     // register-pass program of the plain sweep
     size_t passprog_cap = 0;
+    void *d_scratch = nullptr;      // this device's static scratch array (sq_apply_gates_noalloc)
     int sweep_mode = 0;             // 0: register passes, 1: one launch per gate
     int pass_qbits = 5;             // register qubits per pass (1..5; set by sq_create)
     int lane_moves = 3;             // top lane positions a pass may give to other qubits (0..3)
@@ -117,7 +119,11 @@ struct sq_engine {
     int lower(const sq_gate &g, sq::LoweredGate &lg);
     int apply_lowered(const sq::LoweredGate &g);
     int apply_plain(const std::vector<sq_gate> &gates, const std::vector<sq::LoweredGate> &low);
-    int build_plain(const std::vector<sq_gate> &gates, const std::vector<sq::LoweredGate> &low, PlainProgram &prog);
+    // dst: where the op records go (nullptr: the engine's own growable buffer;
+    // otherwise a caller-owned device buffer of dst_cap bytes, never grown)
+    int build_plain(const std::vector<sq_gate> &gates, const std::vector<sq::LoweredGate> &low, PlainProgram &prog,
+                    void *dst = nullptr, size_t dst_cap = 0);
+    int setup_scratch();
     int run_plain_program(const PlainProgram &prog, const std::vector<sq::LoweredGate> &low);
     int ensure_partials();
     int norm_zz(double *v);

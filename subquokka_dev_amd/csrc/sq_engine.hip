@@ -322,6 +322,16 @@ int create_engine(const sq_config *cfg, const char *const *files, int n_files, b
         delete e;
         return SQ_ERR_OOM;
     }
+    // the device address of the no-allocation entry point's scratch array
+    // (once per device; every engine on the device shares it)
+    if (int rc = e->setup_scratch()) {
+        std::fprintf(stderr, "[sq_create]: %s\n", e->err.c_str());
+        if (e->out_of_core()) e->stream_destroy();
+        if (e->d_state) hipFree(e->d_state);
+        hipStreamDestroy(e->stream);
+        delete e;
+        return rc;
+    }
     int cus = 256;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device);
     e->max_blocks = (unsigned)std::max(1, cus) * 8;

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "subquokka_hip.h"
@@ -130,7 +132,7 @@ int launch_pass(sq_engine *e, const PassArgs<T> &a) {
 
 template <typename T>
 int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low,
-                  PlainProgram &prog) {
+                  PlainProgram &prog, void *dst, size_t dst_cap) {
     constexpr int VB = Traits<T>::VB;
     constexpr int S = VB + 6;
     const int n = e->n;
@@ -245,6 +247,18 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
 
     // upload the program once; earlier launches may still read the old one
     const size_t need = ops.size() * sizeof(PassOp<T>) + 256;
+    if (dst) {   // caller-owned buffer: never grown
+        if (ops.size() * sizeof(PassOp<T>) > dst_cap)
+            return e->fail(SQ_ERR_ARG, "the register-pass program (" + std::to_string(ops.size() * sizeof(PassOp<T>)) +
+                                           " bytes) exceeds the " + std::to_string(dst_cap) + "-byte scratch");
+        prog.d_ops = dst;
+        if (!ops.empty()) {
+            if (hipStreamSynchronize(e->stream) != hipSuccess) return e->fail(SQ_ERR_HIP, "stream sync");
+            hipError_t he = hipMemcpy(dst, ops.data(), ops.size() * sizeof(PassOp<T>), hipMemcpyHostToDevice);
+            if (he != hipSuccess) return e->hip_fail(he, "hipMemcpy(pass program to scratch)");
+        }
+        return SQ_OK;
+    }
     if (!ops.empty()) {
         if (hipStreamSynchronize(e->stream) != hipSuccess) return e->fail(SQ_ERR_HIP, "stream sync");
         if (need > e->passprog_cap) {
@@ -259,11 +273,12 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
         hipError_t he = hipMemcpy(d, ops.data(), ops.size() * sizeof(PassOp<T>), hipMemcpyHostToDevice);
         if (he != hipSuccess) return e->hip_fail(he, "hipMemcpy(pass program)");
     }
+    prog.d_ops = e->d_passprog;
     return SQ_OK;
 }
 
 // Runs a program on the engine's current state buffer (e->n must be the
-// program's n); the program's ops are the last ones uploaded.
+// program's n); its op records are at prog.d_ops.
 template <typename T>
 int run_plain_t(sq_engine *e, const PlainProgram &prog, const std::vector<LoweredGate> &low) {
     constexpr int VB = Traits<T>::VB;
@@ -277,7 +292,7 @@ int run_plain_t(sq_engine *e, const PlainProgram &prog, const std::vector<Lowere
         }
         return SQ_OK;
     }
-    const PassOp<T> *dops = reinterpret_cast<const PassOp<T> *>(e->d_passprog);
+    const PassOp<T> *dops = reinterpret_cast<const PassOp<T> *>(prog.d_ops);
     char nm[48];
     std::snprintf(nm, sizeof nm, "pass%d_%s", kh, e->dtype == SQ_C128 ? "c128" : "c64");
     const double bytes = 2.0 * e->amp_bytes() * std::ldexp(1.0, n);
@@ -317,8 +332,10 @@ int run_plain_t(sq_engine *e, const PlainProgram &prog, const std::vector<Lowere
 
 }  // namespace
 
-int sq_engine::build_plain(const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low, PlainProgram &prog) {
-    return dtype == SQ_C128 ? build_plain_t<double>(this, gates, low, prog) : build_plain_t<float>(this, gates, low, prog);
+int sq_engine::build_plain(const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low, PlainProgram &prog,
+                           void *dst, size_t dst_cap) {
+    return dtype == SQ_C128 ? build_plain_t<double>(this, gates, low, prog, dst, dst_cap)
+                            : build_plain_t<float>(this, gates, low, prog, dst, dst_cap);
 }
 
 int sq_engine::run_plain_program(const PlainProgram &prog, const std::vector<LoweredGate> &low) {
@@ -329,4 +346,58 @@ int sq_engine::apply_plain(const std::vector<sq_gate> &gates, const std::vector<
     PlainProgram prog;
     int rc = build_plain(gates, low, prog);
     return rc ? rc : run_plain_program(prog, low);
+}
+
+// ---------------------------------------------------------------------------
+// sq_apply_gates_noalloc: the plain sweep with its register-pass program in a
+// fixed device array instead of the engine's growable buffer, so the call
+// allocates nothing.  Every engine on a device shares that device's copy of
+// the array; it keeps the last call's program.  Calls that overlap on one
+// device (two host threads, or two engines on different streams) race on it.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)4 << 20;   // 32768 complex128 / 65536 complex64 op records
+constexpr int kScratchDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex scratch_mu;
+void *scratch_addr[kScratchDevices] = {};   // per device: the array's device address
+
+}  // namespace
+
+// Called by sq_create with the engine's device current.  The array's name is
+// a host-side stand-in; its device address (one per device) comes from
+// hipGetSymbolAddress, once per device.
+int sq_engine::setup_scratch() {
+    if (device < 0 || device >= kScratchDevices) return fail(SQ_ERR_ARG, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_addr[device]) {
+        void *p = nullptr;
+        const hipError_t he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (he != hipSuccess || !p) return hip_fail(he != hipSuccess ? he : hipErrorInvalidValue, "hipGetSymbolAddress(scratch)");
+        scratch_addr[device] = p;
+    }
+    d_scratch = scratch_addr[device];
+    return SQ_OK;
+}
+
+extern "C" int sq_apply_gates_noalloc(sq_engine *e, const sq_gate *gates, uint64_t n) {
+    if (!e || (!gates && n)) return SQ_ERR_ARG;
+    if (e->out_of_core()) return e->fail(SQ_ERR_UNSUPPORTED, "sq_apply_gates_noalloc needs an HBM-resident state");
+    if (!e->d_scratch) return e->fail(SQ_ERR_HIP, "scratch array not set up for this device");
+    if (hipSetDevice(e->device) != hipSuccess) return e->fail(SQ_ERR_HIP, "hipSetDevice");
+    // validate everything first so a bad line (or a program too large for the
+    // scratch) leaves the state untouched
+    std::vector<LoweredGate> low(n);
+    for (uint64_t i = 0; i < n; i++) {
+        int rc = e->lower(gates[i], low[i]);
+        if (rc) { e->err += " (gate " + std::to_string(i) + ")"; return rc; }
+    }
+    const std::vector<sq_gate> gl(gates, gates + n);
+    PlainProgram prog;
+    int rc = e->build_plain(gl, low, prog, e->d_scratch, kScratchBytes);
+    return rc ? rc : e->run_plain_program(prog, low);
 }

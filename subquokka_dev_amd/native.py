@@ -69,6 +69,7 @@ _SIGS = {
     "sq_synchronize": (ctypes.c_int, [_P]),
     "sq_last_error": (ctypes.c_char_p, [_P]),
     "sq_apply_gates": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.c_uint64]),
+    "sq_apply_gates_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.c_uint64]),
     "sq_apply_blocks": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64]),
     "sq_apply_fused": (ctypes.c_int, [_P, ctypes.POINTER(SqFused), ctypes.c_uint64]),
     "sq_run_fused": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64]),
@@ -256,6 +257,11 @@ class Engine:
     def apply_gates(self, gates, n=None):
         n = len(gates) if n is None else n
         self._c(lib().sq_apply_gates(self.h, gates, n))
+
+    def apply_gates_noalloc(self, gates, n=None):
+        """sq_apply_gates_noalloc: the same sweep, no device allocation."""
+        n = len(gates) if n is None else n
+        self._c(lib().sq_apply_gates_noalloc(self.h, gates, n))
 
     def apply_text(self, text):
         gates, _, ng, _ = parse_circuit(text, self.n)

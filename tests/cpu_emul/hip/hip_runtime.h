@@ -84,6 +84,12 @@ inline hipError_t hipEventCreate(hipEvent_t *e) { *e = new sq_emul_event; return
 inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+// a __device__ array is an ordinary host array here: its address is itself
+#define HIP_SYMBOL(x) (&(x))
+template <typename S> inline hipError_t hipGetSymbolAddress(void **p, const S *sym) {
+    *p = const_cast<void *>(static_cast<const void *>(sym));
+    return hipSuccess;
+}
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0; return hipSuccess; }
 
 // Matrix-core instructions are wave-wide; here they leave the accumulator as is.
