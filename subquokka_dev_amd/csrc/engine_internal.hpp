@@ -23,9 +23,9 @@ struct PlainLaunch {
     bool merged = false;
     int gate = -1;                 // single-gate step: index into the lowered list
     int op_begin = 0, n_ops = 0;   // merged step: slice of the uploaded ops
-    double flops_per_amp = 0;      // algorithmic real flops per amplitude
+    double flops_per_amp = 0;      // FP64 (complex64: FP32) operations executed per amplitude
     int hpos[5] = {};              // register qubits of the pass (virtual positions)
-    int lh[3] = {};                // qubits in the pass's top three lane positions
+    std::vector<int> v2l;          // virtual position -> qubit (its index bit)
 };
 struct PlainProgram {
     int n = 0, kh = 0;
@@ -70,6 +70,8 @@ struct sq_engine {
     int sweep_mode = 0;             // 0: register passes, 1: one launch per gate
     int pass_qbits = 5;             // register qubits per pass (1..5; set by sq_create)
     int lane_moves = 3;             // top lane positions a pass may give to other qubits (0..3)
+    bool bitperm_ok = true;         // the blocked sweep's final qubit order by one permutation pass
+    void *d_alt = nullptr;          // second state buffer (out-of-place permutation pass)
 
     // out-of-core runner (resident_qbit > 0): state outside HBM
     int res_qbit = 0;
@@ -123,6 +125,10 @@ struct sq_engine {
     // otherwise a caller-owned device buffer of dst_cap bytes, never grown)
     int build_plain(const std::vector<sq_gate> &gates, const std::vector<sq::LoweredGate> &low, PlainProgram &prog,
                     void *dst = nullptr, size_t dst_cap = 0);
+    int ensure_alt();
+    // Moves index bit q of the state to bit perm[q] (a permutation of [0, n)
+    // keeping the vector bits), out of place through d_alt in one pass.
+    int bit_permute(const std::vector<int> &perm);
     int setup_scratch();
     int run_plain_program(const PlainProgram &prog, const std::vector<sq::LoweredGate> &low);
     int ensure_partials();

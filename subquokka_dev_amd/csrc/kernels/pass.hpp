@@ -107,18 +107,24 @@ template <typename T> struct PassRec {
 constexpr int kPassMaxKH = 5;
 constexpr int kPassLaneHi = 3;   // top lane bits a pass may give to other qubits
 constexpr int kPassFoldPhase = 4;   // PassOp<double>::coef slot of PO_DENSE_REG_C's phase
+constexpr int kPassMaxTileBits = 32;
+constexpr int kPassMaxSlotBit = 27;   // a lane's 16-byte slot offset stays below 2^28 (32-bit byte offset)
 
+// The ops work on a virtual index: bits [0, VB) inside the 16-byte vector,
+// [VB, VB+6) the lane, the register slots at hpos[] and the tile number's bits
+// in the remaining positions, ascending.  Where each of those bits lies in
+// the state is given per pass: the qubit a virtual position stands for (the
+// top three lane positions may stand for any qubit) is that qubit's own index
+// bit.  Vector bits never move.
 template <typename T> struct PassArgs {
     const PassOp<T> *ops;
     uint64_t n_tiles;        // wave tiles = 2^(n - VB - 6 - KH)
     int n_ops;
-    int hpos[kPassMaxKH];    // register qubits, ascending, all >= VB + 6
-    // The kernel works on a virtual index: its lane bits VB+3..VB+5 hold the
-    // qubits at physical bits lh[0..2].  Virtual bit VB+3+k and physical bit
-    // lh[k] trade places when a row is addressed (lh[k] = VB+3+k: no trade);
-    // lh[k] - VB <= 27, so a lane's offset stays below 4 GiB.  Everything
-    // else (register qubits, tile bits, op targets) is in virtual positions.
-    int lh[kPassLaneHi];
+    int n_tile_bits;         // log2(n_tiles)
+    int hpos[kPassMaxKH];    // register slots' virtual positions, ascending, all >= VB + 6
+    int lane_pos[6];                     // slot-index bit (index bit - VB) of lane bit k, <= 27
+    int reg_pos[kPassMaxKH];             // index bit of register slot j
+    int tile_pos[kPassMaxTileBits];      // index bit of tile-number bit k
 };
 
 // In-place updates.  The op loop carries the state in fixed registers, so
@@ -168,7 +174,7 @@ template <typename T> __device__ inline void pass_st(pass_gv4f *p, const Pack<T>
 // row kept live across the op loop (2^KH x 2 registers: at KH = 5 those
 // registers are what held the kernel to two waves per SIMD; it runs three).
 template <typename T, int KH>
-__device__ inline pass_gv4f *row_ptr(cx<T> *s, uint64_t tb, const uint64_t (&hb)[KH], int r) {
+__device__ inline pass_gv4f *row_ptr(const cx<T> *s, uint64_t tb, const uint64_t (&hb)[KH], int r) {
     uint64_t off = tb;
 #pragma unroll
     for (int j = 0; j < KH; j++) if ((r >> j) & 1) off += hb[j];
@@ -509,39 +515,35 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
     // apart), where the dealt order ran up to 1.35x slower.
     uint64_t bid = blockIdx.x;
     if ((gridDim.x & 7) == 0) bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
-    const uint64_t wave = bid * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint64_t n_waves = (uint64_t)gridDim.x * (kBlock / 64);
+    // one tile per wave (the grid covers every tile)
+    const uint64_t t = bid * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (t >= a.n_tiles) return;
 
-    // virtual -> physical: swap bit VB+3+k with bit lh[k]
-    auto phys = [&](uint64_t x) {
+    // state addresses of the tile: the tile number's bits placed by tile_pos
+    // (uniform), the register rows by reg_pos, the lane's 16-byte slot by
+    // lane_pos (one 32-bit offset per lane)
+    uint64_t pt = 0;
 #pragma unroll
-        for (int k = 0; k < kPassLaneHi; k++) {
-            const int u = VB + 3 + k, w = a.lh[k];
-            const uint64_t d = ((x >> u) ^ (x >> w)) & 1;
-            x ^= (d << u) | (d << w);
-        }
-        return x;
-    };
+    for (int k = 0; k < kPassMaxTileBits; k++)
+        if (k < a.n_tile_bits) pt |= ((t >> k) & 1) << a.tile_pos[k];
+    pt = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)pt) |
+         ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(pt >> 32)) << 32);
+    uint32_t slot = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) slot |= (uint32_t)((lane >> k) & 1) << a.lane_pos[k];
+    slot &= (1u << (kPassMaxSlotBit + 1)) - 1;
     uint64_t hb[KH];
 #pragma unroll
-    for (int j = 0; j < KH; j++) hb[j] = phys(1ULL << a.hpos[j]);
-    // the lane's 16-byte slot within a row: lane bits 0-2 stay in place, lane
-    // bit 3+k goes to slot bit lh[k] - VB (below 2^28 slots: a 32-bit offset)
-    int lane_slot = lane & 7;
-#pragma unroll
-    for (int k = 0; k < kPassLaneHi; k++) lane_slot |= ((lane >> (3 + k)) & 1) << (a.lh[k] - VB);
-    lane_slot &= (1 << 28) - 1;
-
-    for (uint64_t t = wave; t < a.n_tiles; t += n_waves) {
-        uint64_t tb = t << S;   // tile bits only (uniform)
+    for (int j = 0; j < KH; j++) hb[j] = 1ULL << a.reg_pos[j];
+    {
+        uint64_t tb = t << S;   // virtual tile bits only (uniform)
 #pragma unroll
         for (int j = 0; j < KH; j++) tb = insert_zero(tb, a.hpos[j]);
         const uint64_t base = tb + ((uint64_t)lane << VB);   // virtual index of register 0
-        const uint64_t ptb = phys(tb);
 
         Pack<T> v[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) pass_ld<T>(row_ptr<T, KH>(s, ptb, hb, r) + lane_slot, v[r]);
+        for (int r = 0; r < R; r++) pass_ld<T>(row_ptr<T, KH>(s, pt, hb, r) + slot, v[r]);
 
         for (int op = 0; op < a.n_ops; op++) {
             // every field is read up front: one batch of scalar loads, one wait
@@ -703,7 +705,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
         }
 
 #pragma unroll
-        for (int r = 0; r < R; r++) pass_st<T>(row_ptr<T, KH>(s, ptb, hb, r) + lane_slot, v[r]);
+        for (int r = 0; r < R; r++) pass_st<T>(row_ptr<T, KH>(s, pt, hb, r) + slot, v[r]);
     }
 }
 

@@ -10,9 +10,10 @@
 // (sweep_mode 0) therefore leaves the data where it is: it records the
 // permutation the VSWAPs make (sigma[p] = the index bit that holds the
 // reference's bit p), applies every gate on its mapped targets as the plain
-// sweep does (register passes), and moves the data once at the end, with
-// qubit swaps, so the final layout is exactly the reference's.  A plan whose
-// swaps restore the qubit order moves no data at all.
+// sweep does (register passes), and moves the data once at the end, with one
+// bit-permutation pass (kernels/bitperm.hpp; qubit swaps when no second copy
+// of the state fits in HBM), so the final layout is exactly the reference's.
+// A plan whose swaps restore the qubit order moves no data at all.
 //
 // sweep_mode 1 keeps the reference's shape: VSWAP blocks are permutation
 // sweeps and each gate block is one LDS tile pass (kernels/tile.hpp) over a
@@ -30,6 +31,7 @@
 #include "subquokka_hip.h"
 #include "engine_internal.hpp"
 #include "host/planner.hpp"
+#include "kernels/bitperm.hpp"
 #include "kernels/tile.hpp"
 
 using sq::LoweredGate;
@@ -141,8 +143,17 @@ int run_blocks_mapped(sq_engine *e, const sq_gate *gates, const std::vector<Lowe
     }
     int rc = e->apply_plain(flat, flat_low);
     if (rc) return rc;
-    // move the data to the reference's layout: swap index bits until sigma is
-    // the identity (at most n - 1 swaps)
+    // The reference's bit p is at bit sigma[p]: one permutation pass moves
+    // every bit home (out of place, when a second copy of the state fits)
+    bool moved = false;
+    for (int p = 0; p < n; p++) moved = moved || sigma[p] != p;
+    if (!moved) return SQ_OK;
+    if (e->ensure_alt() == SQ_OK) {
+        std::vector<int> perm(n);
+        for (int p = 0; p < n; p++) perm[sigma[p]] = p;
+        return e->bit_permute(perm);
+    }
+    // otherwise swap index bits until sigma is the identity (at most n - 1 swaps)
     for (int p = 0; p < n; p++) {
         while (sigma[p] != p) {
             const int x = p, y = sigma[p];
@@ -324,6 +335,64 @@ extern "C" int sq_apply_blocks(sq_engine *e, const sq_gate *gates, const uint32_
     if (e->sweep_mode == 0) return run_blocks_mapped(e, gates, low, block_sizes, n_blocks);
     return e->dtype == SQ_C128 ? run_blocks<double>(e, gates, low, block_sizes, n_blocks)
                                : run_blocks<float>(e, gates, low, block_sizes, n_blocks);
+}
+
+int sq_engine::bit_permute(const std::vector<int> &perm) {
+    if ((int)perm.size() != n) return fail(SQ_ERR_ARG, "internal: permutation of the wrong size");
+    std::vector<char> seen(n, 0);
+    for (int q = 0; q < n; q++) {
+        if (perm[q] < 0 || perm[q] >= n || seen[perm[q]]) return fail(SQ_ERR_ARG, "internal: not a permutation");
+        seen[perm[q]] = 1;
+    }
+    int rc = ensure_alt();
+    if (rc) return fail(SQ_ERR_OOM, "no room for the permutation pass's second state buffer");
+    // source-contiguous low bits Ls = [0, c) and the bits that land on the
+    // destination's [0, c); the tile is their union
+    int c = std::min(6, n);
+    std::vector<char> in_t(n, 0);
+    auto tile_bits = [&]() {
+        std::fill(in_t.begin(), in_t.end(), 0);
+        int tb = 0;
+        for (int q = 0; q < n; q++)
+            if (q < c || perm[q] < c) { in_t[q] = 1; tb++; }
+        return tb;
+    };
+    int tb = tile_bits();
+    while (c > 1 && ((size_t)1 << tb) * amp_bytes() > lds_per_block) { c--; tb = tile_bits(); }
+    if (tb > kPermMaxTile || n - tb > kPermMaxBits || ((size_t)1 << tb) * amp_bytes() > lds_per_block)
+        return fail(SQ_ERR_ARG, "internal: permutation tile too wide");
+    BitPermArgs a{};
+    a.tb = tb;
+    a.n_other = n - tb;
+    std::vector<int> load_pos(n, -1);
+    int k = 0;
+    for (int q = 0; q < c; q++) { a.ld_src[k] = q; load_pos[q] = k++; }
+    for (int q = c; q < n; q++) if (in_t[q]) { a.ld_src[k] = q; load_pos[q] = k++; }
+    std::vector<int> inv(n);
+    for (int q = 0; q < n; q++) inv[perm[q]] = q;
+    k = 0;
+    for (int d = 0; d < c; d++) { a.st_loc[k] = load_pos[inv[d]]; a.st_dst[k] = d; k++; }
+    for (int q = 0; q < n; q++)
+        if (in_t[q] && perm[q] >= c) { a.st_loc[k] = load_pos[q]; a.st_dst[k] = perm[q]; k++; }
+    if (k != tb) return fail(SQ_ERR_ARG, "internal: permutation tile bits");
+    k = 0;
+    for (int q = 0; q < n; q++)
+        if (!in_t[q]) { a.other_src[k] = q; a.other_dst[k] = perm[q]; k++; }
+    const uint64_t groups = 1ULL << (n - tb);
+    if (groups > 0x7fffffffULL) return fail(SQ_ERR_ARG, "internal: permutation grid");
+    const size_t lds = ((size_t)1 << tb) * amp_bytes();
+    const int slot = stat_begin(dtype == SQ_C128 ? "bitperm_c128" : "bitperm_c64", 2.0 * (double)state_bytes());
+    if (dtype == SQ_C128)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bitperm<double>), (unsigned)groups, kBlock, lds, stream,
+                           static_cast<const cx<double> *>(d_state), static_cast<cx<double> *>(d_alt), a);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bitperm<float>), (unsigned)groups, kBlock, lds, stream,
+                           static_cast<const cx<float> *>(d_state), static_cast<cx<float> *>(d_alt), a);
+    stat_end(slot);
+    rc = check_launch("k_bitperm");
+    if (rc) return rc;
+    std::swap(d_state, d_alt);   // the state now lives in the other buffer
+    return SQ_OK;
 }
 
 void sq_engine::free_extras() {

@@ -369,6 +369,7 @@ int sq_destroy(sq_engine *e) {
     sq_stats_reset(e);
     if (e->out_of_core()) e->stream_destroy();
     if (e->d_state) hipFree(e->d_state);
+    if (e->d_alt) hipFree(e->d_alt);
     if (e->d_part) hipFree(e->d_part);
     if (e->d_stage) hipFree(e->d_stage);
     e->free_extras();

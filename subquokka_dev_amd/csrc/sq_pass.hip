@@ -122,6 +122,27 @@ void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vec
     ops.push_back(o);
 }
 
+// FP operations the kernel executes per amplitude for one op (an FMA counts
+// two): dense pairs take 14 per amplitude in general, 12 with m0 real and 6
+// with all four coefficients real; a diagonal factor costs 6 per amplitude it
+// multiplies, and register halves, register entries and tile-uniform entries
+// whose factor is 1 are skipped.  A controlled phase folded into a dense op
+// adds nothing.
+template <typename T> double executed_flops(const PassOp<T> &o) {
+    const double quarter = __builtin_popcount(o.active & 15u) / 4.0;
+    switch (o.kind) {
+    case PO_DENSE_REG:
+    case PO_DENSE_REG_C: return o.c == 15 ? 6.0 : (o.c == 1 ? 12.0 : 14.0);
+    case PO_DENSE_LANE:
+    case PO_DENSE_LANE_C:
+    case PO_DENSE_VEC: return 14.0;
+    case PO_DIAG_FREE: return o.c ? 6.0 * quarter : 6.0;
+    case PO_DIAG_REG1: return o.c ? 6.0 * quarter : 3.0 * (((o.active & 3u) != 0) + ((o.active & 12u) != 0));
+    case PO_DIAG_REG2: return 6.0 * quarter;
+    default: return 0.0;
+    }
+}
+
 template <typename T, int KH>
 int launch_pass(sq_engine *e, const PassArgs<T> &a) {
     const unsigned grid = e->grid_flat(a.n_tiles, kBlock / 64);
@@ -178,9 +199,10 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
             const int u = S - kPassLaneHi + k;
             const int off = k - (kPassLaneHi - moves);   // st.lanes covers the top `moves` positions
             const int w = off >= 0 ? st.lanes[off] : u;
-            l.lh[k] = w;
             if (w != u) std::swap(vpos[u], vpos[w]);
         }
+        l.v2l.assign(n, 0);
+        for (int q = 0; q < n; q++) l.v2l[vpos[q]] = q;
         // exactly kh register qubits: pad with qubits outside the lane positions
         std::vector<int> regs;
         for (int q : st.regs) regs.push_back(vpos[q]);
@@ -219,11 +241,7 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
             }
         }
         for (size_t i = 0; i < lg.size(); i++) {
-            const int gi = st.gates[i];
-            if (fold_into[i] >= 0) {
-                l.flops_per_amp += 6.0;   // applied inside the dense gate's op
-                continue;
-            }
+            if (fold_into[i] >= 0) continue;   // applied inside the dense gate's op: no work of its own
             encode_pass_op<T>(lg[i], regs, ops);
             if constexpr (sizeof(T) == 8) if (folded_from[i] >= 0) {
                 const LoweredGate &d = lg[folded_from[i]];
@@ -237,8 +255,7 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
                 o.c = !lane_target && o.coef[0].im == (T)0 ? 1 : 0;
                 o.coef[kPassFoldPhase] = {(T)d.m[3].real(), (T)d.m[3].imag()};
             }
-            // dense 2x2 row: two complex products and a sum; diagonal: one product
-            l.flops_per_amp += low[gi].cls == OpClass::Dense ? 14.0 : 6.0;
+            l.flops_per_amp += executed_flops(ops.back());
         }
         l.n_ops = (int)ops.size() - l.op_begin;
         for (int j = 0; j < kh; j++) l.hpos[j] = regs[j];
@@ -306,13 +323,23 @@ int run_plain_t(sq_engine *e, const PlainProgram &prog, const std::vector<Lowere
         PassArgs<T> a{};
         a.ops = dops + l.op_begin;
         a.n_ops = l.n_ops;
-        a.n_tiles = 1ULL << (n - S - kh);
-        for (int j = 0; j < kh; j++) a.hpos[j] = l.hpos[j];
-        for (int k = 0; k < kPassLaneHi; k++) {
-            // the kernel's lane offset holds slot bits below 28 only
-            if (l.lh[k] < VB + 3 || l.lh[k] >= n || l.lh[k] - VB > 27)
+        a.n_tile_bits = n - S - kh;
+        a.n_tiles = 1ULL << a.n_tile_bits;
+        if ((int)l.v2l.size() != n || a.n_tile_bits > kPassMaxTileBits) return e->fail(SQ_ERR_ARG, "internal: pass layout");
+        // the qubit a virtual position stands for lies at its own index bit
+        for (int j = 0; j < kh; j++) {
+            a.hpos[j] = l.hpos[j];
+            a.reg_pos[j] = l.v2l[l.hpos[j]];
+        }
+        for (int k = 0; k < 6; k++) {
+            // the kernel's lane offset holds slot bits up to kPassMaxSlotBit only
+            a.lane_pos[k] = l.v2l[VB + k] - VB;
+            if (a.lane_pos[k] < 0 || a.lane_pos[k] > kPassMaxSlotBit)
                 return e->fail(SQ_ERR_ARG, "internal: lane position out of range");
-            a.lh[k] = l.lh[k];
+        }
+        for (int v = S, k = 0; v < n; v++) {
+            if (std::find(l.hpos, l.hpos + kh, v) != l.hpos + kh) continue;
+            a.tile_pos[k++] = l.v2l[v];
         }
         int slot = e->stat_begin(nm, bytes, l.flops_per_amp * std::ldexp(1.0, n));
         int rc;
@@ -336,6 +363,23 @@ int sq_engine::build_plain(const std::vector<sq_gate> &gates, const std::vector<
                            void *dst, size_t dst_cap) {
     return dtype == SQ_C128 ? build_plain_t<double>(this, gates, low, prog, dst, dst_cap)
                             : build_plain_t<float>(this, gates, low, prog, dst, dst_cap);
+}
+
+// The second state buffer of the out-of-place permutation pass, allocated
+// once when a second copy of the state fits in HBM with room to spare.
+// SQ_ERR_UNSUPPORTED (no error recorded) when it does not.
+int sq_engine::ensure_alt() {
+    if (d_alt) return SQ_OK;
+    if (out_of_core() || !bitperm_ok) return SQ_ERR_UNSUPPORTED;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { (void)hipGetLastError(); return SQ_ERR_UNSUPPORTED; }
+    if (free_b < state_bytes() + ((size_t)4 << 30)) return SQ_ERR_UNSUPPORTED;
+    if (hipMalloc(&d_alt, state_bytes()) != hipSuccess) {
+        (void)hipGetLastError();
+        d_alt = nullptr;
+        return SQ_ERR_UNSUPPORTED;
+    }
+    return SQ_OK;
 }
 
 int sq_engine::run_plain_program(const PlainProgram &prog, const std::vector<LoweredGate> &low) {

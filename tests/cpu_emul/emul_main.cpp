@@ -72,6 +72,7 @@ int main(int argc, char **argv) {
     // otherwise (then only the addressing is checked, not the values).
     e->lane_moves = 0;
     if (const char *lm = std::getenv("EMUL_LANE_MOVES")) e->lane_moves = std::atoi(lm);
+    if (const char *bp = std::getenv("EMUL_BITPERM")) e->bitperm_ok = std::atoi(bp) != 0;
     if (mode == "adjoint") {   // chunk argument = layers; theta from a fixed formula
         const int layers = chunk;
         std::vector<double> theta((size_t)2 * n * layers), grad(theta.size());

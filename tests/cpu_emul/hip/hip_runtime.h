@@ -59,6 +59,11 @@ inline const char *hipGetErrorString(hipError_t e) { return e ? "emulated error"
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipMemGetInfo(size_t *free_b, size_t *total_b) {
+    *free_b = (size_t)64 << 30;   // plenty: the emulation's states are small
+    *total_b = (size_t)64 << 30;
+    return hipSuccess;
+}
 inline hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int) {
     *v = a == hipDeviceAttributeMultiprocessorCount ? 4 : 65536;
     return hipSuccess;

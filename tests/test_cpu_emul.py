@@ -304,3 +304,28 @@ def test_out_of_core_finder_plan_launches(emul, tmp_path, dtype):
     base = run_case(emul, tmp_path, "oocsub", n, w, dtype, text, seed=2)
     got = run_case(emul, tmp_path, "oocsub", n, w, dtype, text, seed=2, tier="files")
     assert np.array_equal(got, base)
+
+
+@pytest.mark.parametrize("bitperm", [0, 1])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n", [14, 16])
+def test_blocked_sweep_final_permutation(emul, tmp_path, oracle, dtype, n, bitperm):
+    # sweep_mode 0: VSWAP blocks become a relabelling, and the data moves to
+    # the reference's final layout at the end: one bit-permutation pass
+    # through LDS and the second state buffer (bitperm 1), or qubit swaps
+    # (bitperm 0).  Every access stays in bounds for both.  Gates and VSWAPs
+    # act on qubits >= 7 only, so the emulation computes the swap path's
+    # values exactly and they must equal the oracle's subcircuit sweep (it
+    # does not model LDS shared between threads, so the permutation pass's
+    # values are checked on the GPU).
+    from helpers import high_qubit_subcircuit
+    chunk = n - 3
+    text = high_qubit_subcircuit(n, chunk, seed=3 * n + bitperm, n_blocks=9)
+    os.environ["EMUL_BITPERM"] = str(bitperm)
+    try:
+        got = run_case(emul, tmp_path, "sub", n, chunk, dtype, text, seed=n)
+    finally:
+        os.environ.pop("EMUL_BITPERM", None)
+    if not bitperm:
+        want = oracle.run(sq.gen_state(n, n), n, text, chunk=chunk, sub=True)
+        assert np.max(np.abs(got - want)) <= (1e-12 if dtype == "c128" else 2e-6)
