@@ -9,6 +9,7 @@
 
 #include "subquokka_hip.h"
 #include "host/gate_lowering.hpp"
+#include "host/planner.hpp"
 
 struct PendingStat {
     char name[48];
@@ -70,6 +71,15 @@ struct sq_engine {
     int sweep_mode = 0;             // 0: register passes, 1: one launch per gate
     int pass_qbits = 5;             // register qubits per pass (1..5; set by sq_create)
     int lane_moves = 3;             // top lane positions a pass may give to other qubits (0..3)
+    int plan_beam = sq::kDefaultPlanBeam;   // register-pass planner: sets kept per size (1: greedy)
+    // Register-pass schedules of recent gate lists, keyed by their structure
+    // (qubits and gate classes: the schedule does not depend on angles or
+    // coefficients), most recent first.
+    struct PlanCacheEntry {
+        std::vector<int32_t> key;
+        std::vector<sq::SweepStep> steps;
+    };
+    std::vector<PlanCacheEntry> plan_cache;
     bool bitperm_ok = true;         // the blocked sweep's final qubit order by one permutation pass
     void *d_alt = nullptr;          // second state buffer (out-of-place permutation pass)
 

@@ -138,10 +138,10 @@ namespace {
 // run, in order.  Stops once every qubit is blocked.
 int count_pass(const std::vector<sq_gate> &gates, const std::vector<int> &pending, const std::vector<char> &in,
                const std::vector<char> &mergeable, const std::vector<char> &anywhere, std::vector<char> &blocked,
-               std::vector<int> *out) {
+               std::vector<int> *out, int *dense = nullptr) {
     std::fill(blocked.begin(), blocked.end(), 0);
     const int n = (int)blocked.size();
-    int cnt = 0, n_blocked = 0;
+    int cnt = 0, n_blocked = 0, n_dense = 0;
     for (int gi : pending) {
         const sq_gate &g = gates[gi];
         bool ok = mergeable[gi] != 0;
@@ -149,6 +149,7 @@ int count_pass(const std::vector<sq_gate> &gates, const std::vector<int> &pendin
             if (blocked[g.q[i]] || (!anywhere[gi] && !in[g.q[i]])) ok = false;
         if (ok) {
             cnt++;
+            n_dense += !anywhere[gi];
             if (out) out->push_back(gi);
             continue;
         }
@@ -156,6 +157,7 @@ int count_pass(const std::vector<sq_gate> &gates, const std::vector<int> &pendin
             if (!blocked[g.q[i]]) { blocked[g.q[i]] = 1; n_blocked++; }
         if (n_blocked == n) break;
     }
+    if (dense) *dense = n_dense;
     return cnt;
 }
 
@@ -274,7 +276,7 @@ int lane_cost(const std::vector<int> &lanes, const std::vector<int> &dense_cnt, 
 
 std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, const std::vector<char> &mergeable,
                                             const std::vector<char> &anywhere, int n, int low, int kh, int nlh,
-                                            int lh_max) {
+                                            int lh_max, int beam) {
     std::vector<SweepStep> steps;
     std::vector<int> pending(gates.size());
     for (size_t i = 0; i < gates.size(); i++) pending[i] = (int)i;
@@ -293,33 +295,69 @@ std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, c
         }
         std::fill(in.begin(), in.end(), 0);
         for (int q = 0; q < fixed; q++) in[q] = 1;
-        std::vector<int> chosen;
         if (nlh == 0) {
             for (int q = fixed; q < std::min(low, n); q++) in[q] = 1;
         }
         // dense gates per qubit among the pending ones, for the lane choice
         std::vector<int> dense_cnt(n, 0);
-        int best = count_pass(gates, pending, in, mergeable, anywhere, blocked, nullptr);
+        // Beam search over the chosen qubits: keep the `beam` best sets at each
+        // size (most dense gates runnable, then most gates), grow each by every
+        // qubit that still fits the register and lane slots.  beam = 1 is the
+        // finder's greedy rule.
+        struct State {
+            int dense = 0, total = 0;
+            std::vector<int> chosen;
+        };
+        auto better = [](const State &x, const State &y) {
+            return x.dense != y.dense ? x.dense > y.dense : x.total > y.total;
+        };
+        std::vector<State> states(1);
+        states[0].total = count_pass(gates, pending, in, mergeable, anywhere, blocked, nullptr, &states[0].dense);
+        std::vector<char> work(n);
         const int picks = kh + nlh;
-        while ((int)chosen.size() < picks) {
-            int best_q = -1, best_c = best;
-            for (int q = (nlh ? lo : low); q < n; q++) {
-                if (in[q]) continue;
-                if (nlh) {
-                    chosen.push_back(q);
-                    const bool fits = assign_lanes(chosen, dense_cnt, lo, nlh, kh, lh_max, nullptr, nullptr);
-                    chosen.pop_back();
-                    if (!fits) continue;
+        for (int step = 0; step < picks; step++) {
+            std::vector<State> next;
+            std::vector<std::vector<int>> seen;
+            for (const State &cur : states) {
+                for (int q = (nlh ? lo : low); q < n; q++) {
+                    if (in[q] || std::find(cur.chosen.begin(), cur.chosen.end(), q) != cur.chosen.end()) continue;
+                    State ns;
+                    ns.chosen = cur.chosen;
+                    ns.chosen.push_back(q);
+                    if (nlh && !assign_lanes(ns.chosen, dense_cnt, lo, nlh, kh, lh_max, nullptr, nullptr)) continue;
+                    if (beam > 1) {
+                        std::vector<int> key = ns.chosen;
+                        std::sort(key.begin(), key.end());
+                        if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+                        seen.push_back(key);
+                    }
+                    work = in;
+                    for (int x : ns.chosen) work[x] = 1;
+                    ns.total = count_pass(gates, pending, work, mergeable, anywhere, blocked, nullptr, &ns.dense);
+                    next.push_back(std::move(ns));
                 }
-                in[q] = 1;
-                const int c = count_pass(gates, pending, in, mergeable, anywhere, blocked, nullptr);
-                in[q] = 0;
-                if (c > best_c) { best_c = c; best_q = q; }
             }
-            if (best_q < 0) break;
-            in[best_q] = 1;
-            chosen.push_back(best_q);
-            best = best_c;
+            if (next.empty()) break;
+            std::stable_sort(next.begin(), next.end(), better);
+            if ((int)next.size() > std::max(1, beam)) next.resize(std::max(1, beam));
+            if (!better(next[0], states[0]) && beam <= 1) break;   // greedy: stop when no qubit helps
+            states.swap(next);
+        }
+        // qubits that carry no dense gate of the pass take no slot (diagonal
+        // gates run on any qubit)
+        std::vector<int> chosen;
+        {
+            for (int x : states[0].chosen) in[x] = 1;
+            std::vector<int> sel;
+            count_pass(gates, pending, in, mergeable, anywhere, blocked, &sel);
+            std::vector<char> used(n, 0);
+            for (int gi : sel)
+                if (!anywhere[gi])
+                    for (int i = 0; i < gates[gi].nq; i++) used[gates[gi].q[i]] = 1;
+            for (int x : states[0].chosen) {
+                if (used[x]) chosen.push_back(x);
+                else in[x] = 0;
+            }
         }
         SweepStep st;
         st.merged = true;
@@ -475,7 +513,8 @@ extern "C" int sq_plan(const char *circuit_text, int32_t n, int32_t mode, int32_
             if (lg.cls == sq::OpClass::Diag && lg.k <= 2) mergeable[i] = anywhere[i] = 1;
             else if (lg.cls == sq::OpClass::Dense && lg.k == 1) mergeable[i] = 1;
         }
-        for (auto &st : sq::plan_register_passes(gates, mergeable, anywhere, n, 6, param, 3, std::min(n - 1, 27))) {
+        for (auto &st : sq::plan_register_passes(gates, mergeable, anywhere, n, 6, param, 3, std::min(n - 1, 27),
+                                                 sq::kDefaultPlanBeam)) {
             out += std::to_string(st.gates.size()) + "\n";
             for (int gi : st.gates) out += sq::gate_to_line(gates[gi]) + "\n";
         }

@@ -175,8 +175,32 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
     // the top three lane positions may hold any qubit up to bit 27 of the
     // 16-byte slot index (a lane's offset stays a 32-bit one)
     const int moves = std::max(0, std::min(e->lane_moves, kPassLaneHi));
-    const auto steps = sq::plan_register_passes(gates, mergeable, anywhere, n, S, kh, moves,
-                                                std::min(n - 1, 27 + VB));
+    // The schedule depends on the gates' qubits and classes only: a list
+    // with the structure of a recent one (the same circuit again, or with new
+    // angles) reuses its schedule; the op records are encoded every call.
+    std::vector<int32_t> key = {n, S, kh, moves, std::min(n - 1, 27 + VB), e->plan_beam, (int32_t)ng};
+    key.reserve(6 + 6 * ng);
+    for (size_t i = 0; i < ng; i++) {
+        key.push_back(mergeable[i] | (anywhere[i] << 1) | (gates[i].nq << 2));
+        for (int j = 0; j < gates[i].nq && j < 4; j++) key.push_back(gates[i].q[j]);
+    }
+    const std::vector<sq::SweepStep> *steps_ptr = nullptr;
+    for (size_t c = 0; c < e->plan_cache.size(); c++) {
+        if (e->plan_cache[c].key != key) continue;
+        if (c) std::rotate(e->plan_cache.begin(), e->plan_cache.begin() + c, e->plan_cache.begin() + c + 1);
+        steps_ptr = &e->plan_cache[0].steps;
+        break;
+    }
+    if (!steps_ptr) {
+        sq_engine::PlanCacheEntry ent;
+        ent.steps = sq::plan_register_passes(gates, mergeable, anywhere, n, S, kh, moves, std::min(n - 1, 27 + VB),
+                                             e->plan_beam);
+        ent.key = std::move(key);
+        e->plan_cache.insert(e->plan_cache.begin(), std::move(ent));
+        if (e->plan_cache.size() > 4) e->plan_cache.pop_back();
+        steps_ptr = &e->plan_cache[0].steps;
+    }
+    const std::vector<sq::SweepStep> &steps = *steps_ptr;
 
     std::vector<PassOp<T>> ops;
     for (const auto &st : steps) {

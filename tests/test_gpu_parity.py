@@ -354,3 +354,28 @@ def test_finder_plan_vswap_blocks(oracle, n, chunk, sweep_mode):
     say(f"finder plan n={n} chunk={chunk}")
     check(run_gpu(n, plan, s0, "c128", sub=True, chunk=chunk, sweep_mode=sweep_mode), oracle.run(s0, n, text), "c128",
           "finder plan")
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_schedule_reused_for_the_same_structure(oracle, dtype):
+    # the register-pass schedule is cached by the gates' qubits and classes:
+    # the same circuit with other angles reuses it, with its own coefficients
+    n = 16
+    a = sq.gen_circuit("random", n, 6, seed=11)
+    b = sq.gen_circuit("random", n, 6, seed=12)   # another matching and other angles
+    a2 = "".join(line.split()[0] + " " + " ".join(line.split()[1:2]) + " " +
+                 " ".join(repr(float(x) * 0.5) for x in line.split()[2:]) + "\n"
+                 if line.startswith("U1") else line + "\n" for line in a.strip().splitlines())
+    s0 = random_state(n, 4)
+    tol = 1e-12 if dtype == "c128" else 2e-6
+    eng = sq.Engine(n, dtype=dtype)
+    try:
+        state = s0
+        for text in (a, a2, b, a):
+            eng.set_state(state.astype(np.complex128 if dtype == "c128" else np.complex64))
+            eng.apply_text(text)
+            want = oracle.run(state, n, text)
+            assert np.max(np.abs(eng.get_state() - want)) <= tol
+            state = want
+    finally:
+        eng.close()
