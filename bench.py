@@ -591,8 +591,51 @@ def main():
         ea.close()
         f, gtime = min(fwd), min(grd)
         out["adjoint_24q_384p_c64"] = {"forward_ms": round(f * 1e3, 3), "energy_and_gradient_ms": round(gtime * 1e3, 3),
-                                       "gradient_over_forward": round(gtime / f, 3), "checkpoint_every_layers": 3}
+                                       "gradient_over_forward": round(gtime / f, 3), "checkpoint_every_layers": 3,
+                                       "entry": "sq_adjoint_grad_hea (the ansatz's own kernels)"}
         print(f"[bench] adjoint 24q: forward {f*1e3:.2f} ms, energy+gradient {gtime*1e3:.2f} ms",
+              file=sys.stderr, flush=True)
+        # The same ansatz as circuit text through the general entry point
+        # (sq_adjoint_grad; CNOT as U2): its forward is the plain sweep's
+        # register passes, against which the gradient is timed.
+        atext = sq.gen_circuit("hea", an, al, theta=theta)
+        alines = atext.strip().splitlines()
+        aparam, k = [], 0
+        for line in alines:
+            if line.startswith(("RY", "RZ")):
+                aparam.append(k)
+                k += 1
+            else:
+                aparam.append(-1)
+        aterms = [(1.0, [q, (q + 1) % an]) for q in range(an)]
+        ea = sq.Engine(an, dtype="c64")
+        agates, _, ang, _ = sq.parse_circuit(atext, an)
+        ea.init_zero()
+        ea.apply_gates(agates, ang)
+        ea.adjoint_grad(atext, aparam, theta, aterms, 144)
+        ea.sync()
+        fwd, grd = [], []
+        for _ in range(3):
+            ea.init_zero()
+            ea.sync()
+            t0 = time.perf_counter()
+            ea.apply_gates(agates, ang)
+            e_fwd = ea.expect_z(aterms)
+            fwd.append(time.perf_counter() - t0)
+            ea.init_zero()
+            ea.sync()
+            t0 = time.perf_counter()
+            e_g, g2 = ea.adjoint_grad(atext, aparam, theta, aterms, 144)
+            grd.append(time.perf_counter() - t0)
+        ea.close()
+        f2, gt2 = min(fwd), min(grd)
+        out["adjoint_general_24q_384p_c64"] = {
+            "forward_ms": round(f2 * 1e3, 3), "energy_and_gradient_ms": round(gt2 * 1e3, 3),
+            "gradient_over_forward": round(gt2 / f2, 3), "checkpoint_every_params": 144,
+            "energy_diff_vs_forward": abs(e_g - e_fwd),
+            "max_grad_diff_vs_hea_entry": float(np.max(np.abs(g2 - g))),
+            "entry": "sq_adjoint_grad (circuit text, Z-string observable); forward = register passes"}
+        print(f"[bench] adjoint (general) 24q: forward {f2*1e3:.2f} ms, energy+gradient {gt2*1e3:.2f} ms",
               file=sys.stderr, flush=True)
 
     if not args.no_cpu_baseline:

@@ -201,6 +201,33 @@ int sq_hea_energy(sq_engine *e, int32_t layers, const double *theta, uint64_t n_
 int sq_adjoint_grad_hea(sq_engine *e, int32_t layers, const double *theta, uint64_t n_theta,
                         int32_t checkpoint_every, double *energy, double *grad);
 
+/* ---- adjoint gradient of parameterised circuits (no reference counterpart;
+ * the rotation and phase matrices are the reference's, circuit.cpp:2655-2795,
+ * 2947-3041) ----
+ * A diagonal observable is a sum of Z strings: h(i) = sum_t coef_t *
+ * (-1)^popcount(i & mask_t) (mask 0: a constant term). */
+typedef struct sq_zterm {
+    double coef;
+    uint64_t mask;
+} sq_zterm;
+/* <psi|H|psi> of the current state, accumulated in FP64. */
+int sq_expect_z(sq_engine *e, const sq_zterm *terms, uint64_t n_terms, double *out);
+/* Energy E = <psi|H|psi> of the circuit applied to the engine's current state,
+ * and dE/dtheta.  Gate i uses the angle theta[param[i]] when param[i] >= 0
+ * (RX, RY, RZ, P, CP or RZZ; its own param field is ignored; several gates may
+ * share a parameter), else it is a fixed gate of any kind but VSWAP.  The
+ * forward runs as the plain sweep's register passes and keeps an FP64 copy of
+ * the state every checkpoint_every parameterised gates (0: none); the reverse
+ * sweep un-applies every gate to psi and lambda = H psi (fixed gates as
+ * register passes, each parameterised gate in one pass that also takes its
+ * gradient term, accumulated in FP64) and re-anchors psi to the copies.  grad
+ * receives n_theta values.  On return the state is the one the call started
+ * from (exactly, from the first copy, when checkpoint_every > 0; else up to
+ * the rounding of the reverse sweep). */
+int sq_adjoint_grad(sq_engine *e, const sq_gate *gates, const int32_t *param, uint64_t n_gates,
+                    const double *theta, uint64_t n_theta, const sq_zterm *terms, uint64_t n_terms,
+                    int32_t checkpoint_every, double *energy, double *grad);
+
 /* ---- per-kernel timing (HIP events on the engine's stream) ---- */
 typedef struct sq_kstat {
     char name[48];

@@ -5,4 +5,4 @@ include/subquokka_hip.h, plus a C++ host with the reference's INI/circuit
 formats and runner selection).  This package only loads that library.
 """
 from .native import (Engine, SqError, gen_circuit, gen_state, parse_circuit, plan, lib,  # noqa: F401
-                     declared_symbols)
+                     declared_symbols, zterms)

@@ -1,6 +1,12 @@
 """torch.autograd.Function over the engine's adjoint gradient.
 
     energy = HEAEnergy.apply(theta, engine, layers, checkpoint_every)
+    energy = CircuitEnergy.apply(theta, engine, text, param, terms, checkpoint_every)
+
+CircuitEnergy: any parameterised circuit (circuit text, param[i] = index of
+gate i's angle in theta or -1) from |0...0>, observable a sum of Z strings
+[(coef, qubits), ...]; one sq_adjoint_grad call gives the energy and the whole
+gradient.
 
 theta is a 1-D float64 tensor of 2 * n * layers angles (RY angles then RZ
 angles, layer by layer); engine is a subquokka_dev_amd.Engine of n qubits.
@@ -29,3 +35,22 @@ class HEAEnergy(torch.autograd.Function):
 
 def hea_energy(theta, engine, layers, checkpoint_every=0):
     return HEAEnergy.apply(theta, engine, layers, checkpoint_every)
+
+
+class CircuitEnergy(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta, engine, text, param, terms, checkpoint_every=0):
+        th = theta.detach().to(torch.float64).cpu().numpy()
+        engine.init_zero()
+        energy, grad = engine.adjoint_grad(text, param, th, terms, checkpoint_every)
+        ctx.save_for_backward(torch.from_numpy(np.ascontiguousarray(grad)).to(theta.device, theta.dtype))
+        return torch.tensor(energy, dtype=theta.dtype, device=theta.device)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        (grad,) = ctx.saved_tensors
+        return grad_output * grad, None, None, None, None, None
+
+
+def circuit_energy(theta, engine, text, param, terms, checkpoint_every=0):
+    return CircuitEnergy.apply(theta, engine, text, param, terms, checkpoint_every)

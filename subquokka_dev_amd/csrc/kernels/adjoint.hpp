@@ -103,4 +103,119 @@ __global__ void __launch_bounds__(kBlock) k_apply_zz_ring(const cx<T> *__restric
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// General parameterised circuits: one kernel per parameterised gate takes its
+// gradient term and un-applies it on psi and lambda in one pass.
+//   RX, RY, RZ, RZZ:  G = exp(-i t P / 2)  ->  dE/dt = Im <lambda|P|psi>
+//   P, CP:            G = exp(i t Pi)      ->  dE/dt = -2 Im <lambda|Pi|psi>
+// (Pi the projector on |1> / |11>), with psi the state after G and lambda the
+// observable applied to the final state and un-applied down to G.
+// ---------------------------------------------------------------------------
+
+enum AdjKind : int32_t { ADJ_RX = 0, ADJ_RY = 1, ADJ_RZ = 2, ADJ_P = 3, ADJ_CP = 4, ADJ_RZZ = 5 };
+
+struct AdjGateArgs {
+    uint64_t n_items;   // pairs (one-qubit kinds) or amplitudes (CP, RZZ)
+    int kind;
+    int q0, q1;         // qubits (q1: second qubit of CP / RZZ)
+    double c, s;        // cos, sin of the half angle (RX RY RZ RZZ) or of the angle (P CP)
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_adj_gate(cx<T> *__restrict__ psi, cx<T> *__restrict__ lam,
+                                                     const AdjGateArgs a, double *__restrict__ part) {
+    double acc[1] = {0.0};
+    const uint64_t gstride = (uint64_t)gridDim.x * kBlock;
+    const T c = (T)a.c, s = (T)a.s;
+    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < a.n_items; p += gstride) {
+        if (a.kind == ADJ_CP || a.kind == ADJ_RZZ) {
+            const uint64_t i = p;
+            const int b0 = (int)((i >> a.q0) & 1), b1 = (int)((i >> a.q1) & 1);
+            const cx<T> x = psi[i], l = lam[i];
+            // Im(conj(l) x)
+            const double im = (double)l.re * (double)x.im - (double)l.im * (double)x.re;
+            if (a.kind == ADJ_CP) {
+                if (b0 & b1) {
+                    acc[0] += -2.0 * im;
+                    const cx<T> e = {c, -s};   // e^{-i t}
+                    psi[i] = cmul(x, e);
+                    lam[i] = cmul(l, e);
+                }
+            } else {
+                const int z = (b0 == b1) ? 1 : -1;
+                acc[0] += z * im;
+                const cx<T> e = {c, z > 0 ? s : -s};   // RZZ^dag = e^{i t z / 2}
+                psi[i] = cmul(x, e);
+                lam[i] = cmul(l, e);
+            }
+            continue;
+        }
+        const uint64_t i0 = insert_zero(p, a.q0), i1 = i0 | (1ULL << a.q0);
+        const cx<T> x0 = psi[i0], x1 = psi[i1], l0 = lam[i0], l1 = lam[i1];
+        const double x0r = x0.re, x0i = x0.im, x1r = x1.re, x1i = x1.im;
+        const double l0r = l0.re, l0i = l0.im, l1r = l1.re, l1i = l1.im;
+        switch (a.kind) {
+        case ADJ_RX: {
+            // <l|X x> = conj(l0) x1 + conj(l1) x0
+            acc[0] += (l0r * x1i - l0i * x1r) + (l1r * x0i - l1i * x0r);
+            // RX(t)^dag = [[c, i s], [i s, c]]
+            psi[i0] = {c * x0.re - s * x1.im, c * x0.im + s * x1.re};
+            psi[i1] = {c * x1.re - s * x0.im, c * x1.im + s * x0.re};
+            lam[i0] = {c * l0.re - s * l1.im, c * l0.im + s * l1.re};
+            lam[i1] = {c * l1.re - s * l0.im, c * l1.im + s * l0.re};
+            break; }
+        case ADJ_RY: {
+            // <l|Y x> = conj(l0)(-i x1) + conj(l1)(i x0)
+            acc[0] += (l1r * x0r + l1i * x0i) - (l0r * x1r + l0i * x1i);
+            // RY(t)^dag = [[c, s], [-s, c]]
+            psi[i0] = {c * x0.re + s * x1.re, c * x0.im + s * x1.im};
+            psi[i1] = {c * x1.re - s * x0.re, c * x1.im - s * x0.im};
+            lam[i0] = {c * l0.re + s * l1.re, c * l0.im + s * l1.im};
+            lam[i1] = {c * l1.re - s * l0.re, c * l1.im - s * l0.im};
+            break; }
+        case ADJ_RZ: {
+            // <l|Z x> = conj(l0) x0 - conj(l1) x1
+            acc[0] += (l0r * x0i - l0i * x0r) - (l1r * x1i - l1i * x1r);
+            // RZ(t)^dag = diag(e^{i t/2}, e^{-i t/2})
+            const cx<T> ep = {c, s}, en = {c, -s};
+            psi[i0] = cmul(x0, ep);
+            psi[i1] = cmul(x1, en);
+            lam[i0] = cmul(l0, ep);
+            lam[i1] = cmul(l1, en);
+            break; }
+        default: {   // ADJ_P: dE/dt = -2 Im(conj(l1) x1); P(t)^dag = diag(1, e^{-i t})
+            acc[0] += -2.0 * (l1r * x1i - l1i * x1r);
+            const cx<T> e = {c, -s};
+            psi[i1] = cmul(x1, e);
+            lam[i1] = cmul(l1, e);
+            break; }
+        }
+    }
+    block_sum<1>(acc, part + blockIdx.x);
+}
+
+// A diagonal observable as Z strings: h(i) = sum_t coef_t (-1)^{popcount(i & mask_t)}.
+struct ZTerm {
+    double coef;
+    uint64_t mask;
+};
+
+// lambda = H psi, and the block partials of <psi|H|psi> (FP64).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_apply_zterms(const cx<T> *__restrict__ psi, cx<T> *__restrict__ lam,
+                                                         const ZTerm *__restrict__ terms, int n_terms, uint64_t n_amp,
+                                                         double *__restrict__ part) {
+    double acc[1] = {0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n_amp; i += stride) {
+        double h = 0.0;
+        for (int t = 0; t < n_terms; t++) h += (__popcll(i & terms[t].mask) & 1) ? -terms[t].coef : terms[t].coef;
+        const cx<T> x = psi[i];
+        acc[0] += h * ((double)x.re * (double)x.re + (double)x.im * (double)x.im);
+        if (lam) lam[i] = {(T)(h * (double)x.re), (T)(h * (double)x.im)};
+    }
+    block_sum<1>(acc, part + blockIdx.x);
+}
+
 }  // namespace sqk

@@ -45,6 +45,10 @@ class SqConfig(ctypes.Structure):
                 ("pass_qbits", ctypes.c_int32), ("host_tier", ctypes.c_int32)]
 
 
+class SqZTerm(ctypes.Structure):
+    _fields_ = [("coef", ctypes.c_double), ("mask", ctypes.c_uint64)]
+
+
 class SqKstat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double),
                 ("alg_bytes", ctypes.c_double), ("alg_flops", ctypes.c_double)]
@@ -85,6 +89,11 @@ _SIGS = {
                                            ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_double)]),
     "sq_sample": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _P]),
+    "sq_expect_z": (ctypes.c_int, [_P, ctypes.POINTER(SqZTerm), ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
+    "sq_adjoint_grad": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.POINTER(ctypes.c_int32), ctypes.c_uint64,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.c_uint64, ctypes.POINTER(SqZTerm),
+                                       ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]),
     "sq_stats_enable": (ctypes.c_int, [_P, ctypes.c_int32]),
     "sq_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(SqKstat), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "sq_stats_reset": (ctypes.c_int, [_P]),
@@ -180,6 +189,19 @@ def parse_circuit(text, n, is_subcircuit=False):
     if rc != SQ_OK:
         raise SqError(rc, f"parse error at line {el.value}")
     return gates, (blocks if is_subcircuit else None), ng.value, nb.value
+
+
+def zterms(terms):
+    """Z-string observable: [(coef, qubits), ...] -> ctypes array of sq_zterm
+    (h(i) = sum coef * (-1)^(parity of i's bits on qubits))."""
+    arr = (SqZTerm * max(1, len(terms)))()
+    for k, (coef, qubits) in enumerate(terms):
+        mask = 0
+        for q in qubits:
+            mask |= 1 << int(q)
+        arr[k].coef = float(coef)
+        arr[k].mask = mask
+    return arr, len(terms)
 
 
 def plan(text, n, mode, param):
@@ -347,6 +369,33 @@ class Engine:
         return {arr[i].name.decode(): dict(launches=arr[i].launches, total_ms=arr[i].total_ms,
                                            alg_bytes=arr[i].alg_bytes, alg_flops=arr[i].alg_flops)
                 for i in range(n.value)}
+
+    def expect_z(self, terms):
+        """<psi|H|psi> for H = sum coef * prod Z over qubits (terms as for zterms)."""
+        arr, nt = zterms(terms)
+        v = ctypes.c_double()
+        self._c(lib().sq_expect_z(self.h, arr, nt, ctypes.byref(v)))
+        return v.value
+
+    def adjoint_grad(self, text, param, theta, terms, checkpoint_every=0):
+        """Energy and gradient of a parameterised circuit (sq_adjoint_grad) from
+        the current state.  text: circuit lines; param[i] = index into theta of
+        gate i's angle, or -1 for a fixed gate."""
+        gates, _, ng, _ = parse_circuit(text, self.n)
+        pa = np.ascontiguousarray(param, dtype=np.int32).reshape(-1)
+        if pa.size != ng:
+            raise ValueError(f"param has {pa.size} entries for {ng} gates")
+        th = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+        if pa.size and pa.max() >= th.size:
+            raise ValueError("a parameter index lies beyond theta")
+        arr, nt = zterms(terms)
+        g = np.zeros(max(1, th.size))
+        v = ctypes.c_double()
+        self._c(lib().sq_adjoint_grad(self.h, gates, pa.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ng,
+                                      th.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), th.size, arr, nt,
+                                      checkpoint_every, ctypes.byref(v),
+                                      g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return v.value, g[:th.size]
 
     def _hea_theta(self, layers, theta):
         th = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)

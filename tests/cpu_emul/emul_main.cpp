@@ -39,7 +39,7 @@ int main(int argc, char **argv) {
                               sub ? nb : 0, &nb, &line);
     if (rc) { std::fprintf(stderr, "parse rc=%d line %d\n", rc, line); return 3; }
     sq_config cfg{};
-    cfg.total_qbit = n; cfg.chunk_qbit = ooc ? 0 : chunk; cfg.dtype = dtype;
+    cfg.total_qbit = n; cfg.chunk_qbit = (ooc || mode == "adjgen") ? 0 : chunk; cfg.dtype = dtype;
     if (ooc) cfg.resident_qbit = chunk;
     if (const char *kh = std::getenv("EMUL_PASS_QBITS")) cfg.pass_qbits = std::atoi(kh);
     if (const char *sm = std::getenv("EMUL_SWEEP_MODE")) cfg.sweep_mode = std::atoi(sm);
@@ -85,6 +85,28 @@ int main(int argc, char **argv) {
         if (rc) return 11;
         sq_destroy(e);
         std::printf("ok adjoint params=%zu\n", theta.size());
+        return 0;
+    }
+    if (mode == "adjgen") {   // chunk argument = checkpoint_every; every RX RY RZ P CP RZZ takes an angle
+        std::vector<int32_t> param(ng, -1);
+        std::vector<double> theta;
+        for (uint64_t i = 0; i < ng; i++) {
+            const int k = gates[i].kind;
+            if (k == SQ_GATE_RX || k == SQ_GATE_RY || k == SQ_GATE_RZ || k == SQ_GATE_P || k == SQ_GATE_CP || k == SQ_GATE_RZZ) {
+                param[i] = (int32_t)theta.size();
+                theta.push_back(0.1 + 0.37 * (double)i);
+            }
+        }
+        std::vector<sq_zterm> terms;
+        for (int q = 0; q < n; q++) terms.push_back({1.0 + q, (1ULL << q) | (1ULL << ((q + 1) % n))});
+        std::vector<double> grad(theta.size() + 1);
+        double energy = 0;
+        rc = sq_adjoint_grad(e, gates.data(), param.data(), ng, theta.data(), theta.size(), terms.data(), terms.size(),
+                             chunk, &energy, grad.data());
+        if (rc) { std::fprintf(stderr, "adjgen rc=%d %s\n", rc, sq_last_error(e)); return 15; }
+        if ((rc = sq_expect_z(e, terms.data(), terms.size(), &energy))) return 16;
+        sq_destroy(e);
+        std::printf("ok adjgen params=%zu\n", theta.size());
         return 0;
     }
     if (seed_env) {   // seeded Gaussian initial state instead of |0...0>

@@ -329,3 +329,19 @@ def test_blocked_sweep_final_permutation(emul, tmp_path, oracle, dtype, n, bitpe
     if not bitperm:
         want = oracle.run(sq.gen_state(n, n), n, text, chunk=chunk, sub=True)
         assert np.max(np.abs(got - want)) <= (1e-12 if dtype == "c128" else 2e-6)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("n,ckpt", [(4, 0), (11, 3), (13, 7)])
+def test_general_adjoint_launches(emul, tmp_path, dtype, n, ckpt):
+    # sq_adjoint_grad on a circuit of every gate kind (rotations and phases
+    # take angles): the forward's register passes, the checkpoints, the
+    # reverse sweep's kernels and register passes on both vectors stay in
+    # bounds (values are checked on the GPU)
+    text = random_circuit(n, 160, seed=n + ckpt)
+    f = tmp_path / "c.txt"
+    f.write_text(text)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    r = subprocess.run([emul, "adjgen", str(n), str(ckpt), dtype, str(f)], capture_output=True, text=True,
+                       timeout=900, env=env)
+    assert r.returncode == 0 and r.stdout.startswith("ok adjgen"), (r.stdout + r.stderr)[-3000:]
