@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "gate_lowering.hpp"
@@ -274,6 +275,40 @@ int lane_cost(const std::vector<int> &lanes, const std::vector<int> &dense_cnt, 
 
 }  // namespace
 
+namespace {
+
+// count_pass on qubit bit masks (n <= 64), for the planner's inner loop: the
+// gates of `pending` a pass with resident set `in` can run, stopping once
+// every qubit is blocked.
+struct MaskCounter {
+    std::vector<uint64_t> qmask;   // per gate: its qubits
+    const std::vector<char> &mergeable, &anywhere;
+    uint64_t all;
+    MaskCounter(const std::vector<sq_gate> &gates, const std::vector<char> &m, const std::vector<char> &a, int n)
+        : qmask(gates.size()), mergeable(m), anywhere(a), all(n >= 64 ? ~0ULL : (1ULL << n) - 1) {
+        for (size_t i = 0; i < gates.size(); i++)
+            for (int j = 0; j < gates[i].nq; j++) qmask[i] |= 1ULL << gates[i].q[j];
+    }
+    int count(const std::vector<int> &pending, uint64_t in, int *dense) const {
+        uint64_t blocked = 0;
+        int cnt = 0, n_dense = 0;
+        for (int gi : pending) {
+            const uint64_t m = qmask[gi];
+            if (mergeable[gi] && !(m & blocked) && (anywhere[gi] || !(m & ~in))) {
+                cnt++;
+                n_dense += !anywhere[gi];
+                continue;
+            }
+            blocked |= m;
+            if (blocked == all) break;
+        }
+        *dense = n_dense;
+        return cnt;
+    }
+};
+
+}  // namespace
+
 std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, const std::vector<char> &mergeable,
                                             const std::vector<char> &anywhere, int n, int low, int kh, int nlh,
                                             int lh_max, int beam) {
@@ -284,6 +319,20 @@ std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, c
     if (nlh < 0 || nlh > low) nlh = 0;
     const int fixed = std::min(low - nlh, n);
     const int lo = low - nlh;
+    const MaskCounter counter(gates, mergeable, anywhere, n);
+    const uint64_t lane_pos = nlh ? (((1ULL << nlh) - 1) << lo) : 0;
+    const uint64_t lane_ok = lh_max >= 63 ? ~0ULL : (lh_max < 0 ? 0 : (2ULL << lh_max) - 1);
+    // Whether a chosen set fits kh register slots plus the free lane positions
+    // (the test assign_lanes makes, on masks): qubits already in a lane position
+    // stay there, and the excess over kh needs free positions and qubits no
+    // higher than lh_max.
+    auto fits = [&](uint64_t chosen) {
+        const int in_lane = __builtin_popcountll(chosen & lane_pos);
+        const uint64_t rest = chosen & ~lane_pos;
+        const int need = __builtin_popcountll(rest) - kh;
+        if (need <= 0) return true;
+        return nlh - in_lane >= need && __builtin_popcountll(rest & lane_ok) >= need;
+    };
     while (!pending.empty()) {
         const int g0 = pending.front();
         if (!mergeable[g0]) {
@@ -298,6 +347,8 @@ std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, c
         if (nlh == 0) {
             for (int q = fixed; q < std::min(low, n); q++) in[q] = 1;
         }
+        uint64_t in_mask = 0;
+        for (int q = 0; q < n; q++) if (in[q]) in_mask |= 1ULL << q;
         // dense gates per qubit among the pending ones, for the lane choice
         std::vector<int> dense_cnt(n, 0);
         // Beam search over the chosen qubits: keep the `beam` best sets at each
@@ -306,34 +357,31 @@ std::vector<SweepStep> plan_register_passes(const std::vector<sq_gate> &gates, c
         // finder's greedy rule.
         struct State {
             int dense = 0, total = 0;
+            uint64_t mask = 0;
             std::vector<int> chosen;
         };
         auto better = [](const State &x, const State &y) {
             return x.dense != y.dense ? x.dense > y.dense : x.total > y.total;
         };
         std::vector<State> states(1);
-        states[0].total = count_pass(gates, pending, in, mergeable, anywhere, blocked, nullptr, &states[0].dense);
-        std::vector<char> work(n);
+        states[0].total = counter.count(pending, in_mask, &states[0].dense);
         const int picks = kh + nlh;
+        std::unordered_set<uint64_t> seen;
         for (int step = 0; step < picks; step++) {
             std::vector<State> next;
-            std::vector<std::vector<int>> seen;
+            seen.clear();
             for (const State &cur : states) {
                 for (int q = (nlh ? lo : low); q < n; q++) {
-                    if (in[q] || std::find(cur.chosen.begin(), cur.chosen.end(), q) != cur.chosen.end()) continue;
+                    const uint64_t bit = 1ULL << q;
+                    if ((in_mask | cur.mask) & bit) continue;
+                    const uint64_t m = cur.mask | bit;
+                    if (nlh && !fits(m)) continue;
+                    if (beam > 1 && !seen.insert(m).second) continue;
                     State ns;
+                    ns.mask = m;
+                    ns.total = counter.count(pending, in_mask | m, &ns.dense);
                     ns.chosen = cur.chosen;
                     ns.chosen.push_back(q);
-                    if (nlh && !assign_lanes(ns.chosen, dense_cnt, lo, nlh, kh, lh_max, nullptr, nullptr)) continue;
-                    if (beam > 1) {
-                        std::vector<int> key = ns.chosen;
-                        std::sort(key.begin(), key.end());
-                        if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
-                        seen.push_back(key);
-                    }
-                    work = in;
-                    for (int x : ns.chosen) work[x] = 1;
-                    ns.total = count_pass(gates, pending, work, mergeable, anywhere, blocked, nullptr, &ns.dense);
                     next.push_back(std::move(ns));
                 }
             }
