@@ -168,6 +168,17 @@ int sq_apply_fused(sq_engine *e, const sq_fused *blocks, uint64_t n);
  * FP64 on the GPU, then applied on MFMA (FP32 accumulation for SQ_C64, FP64 for
  * SQ_C128).  The result equals applying the gates in order. */
 int sq_run_fused(sq_engine *e, const sq_gate *gates, const uint32_t *block_sizes, uint64_t n_blocks);
+/* A plain gate list (the sq_apply_gates / MEM_Runner::run(vector<Gate*>&) input,
+ * MEM_Runner.cpp:101-134) as fused passes on a complex64 state: the engine cuts
+ * the list into passes over tiles of tile_qbit qubits (0: 12; else 9..13, at
+ * most N), groups each pass's gates into four-qubit blocks, composes each block
+ * in FP64 on the GPU and applies it to the tile in LDS on MFMA with split bf16
+ * products accumulated in FP32 (about 1e-5 relative per block; the reference's
+ * cache-resident subcircuits, MEM_Runner.cpp:173-182, in one read and write of
+ * the state per pass).  Diagonal gates may act on any qubits.  No reference
+ * counterpart for the arithmetic.  SQ_ERR_UNSUPPORTED for SQ_C128 engines and
+ * the out-of-core runner; SQ_ERR_ARG below 9 qubits. */
+int sq_apply_gates_fused(sq_engine *e, const sq_gate *gates, uint64_t n, int32_t tile_qbit);
 /* Composes gates (all targets inside q[0..k-1]) into one 2^k x 2^k FP64 matrix on
  * the GPU; u_out receives 2*4^k doubles. */
 int sq_compose_block(sq_engine *e, const sq_gate *gates, uint64_t n, int32_t k,

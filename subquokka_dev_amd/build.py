@@ -33,7 +33,9 @@ INCLUDES = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 # blocks whose merges copy the whole register-resident state once or twice per
 # op.  Skipping structurization of uniform regions keeps them as plain scalar
 # branches (divergent regions are still structurized).
-SOURCE_FLAGS = {"sq_pass.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions"]}
+SOURCE_FLAGS = {"sq_pass.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions"],
+                # MFMA results in vector registers: no accumulator-register reads per batch
+                "sq_fused.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 HIP_SOURCES = ["sq_engine.hip", "sq_blocks.hip", "sq_fused.hip", "sq_adjoint.hip", "sq_stream.hip", "sq_pass.hip",
                "sq_sample.hip"]

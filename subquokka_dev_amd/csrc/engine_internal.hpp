@@ -4,6 +4,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -28,6 +29,8 @@ struct PlainLaunch {
     int hpos[5] = {};              // register qubits of the pass (virtual positions)
     std::vector<int> v2l;          // virtual position -> qubit (its index bit)
 };
+struct FusedPassPlan;   // fused passes of a gate list (sq_fused.hip)
+
 struct PlainProgram {
     int n = 0, kh = 0;
     bool per_gate = false;         // one launch per gate (sweep_mode 1, small states)
@@ -80,6 +83,8 @@ struct sq_engine {
         std::vector<sq::SweepStep> steps;
     };
     std::vector<PlanCacheEntry> plan_cache;
+    // fused-pass plans of recent gate lists, keyed the same way, most recent first
+    std::vector<std::pair<std::vector<int32_t>, std::shared_ptr<FusedPassPlan>>> fpass_cache;
     bool bitperm_ok = true;         // the blocked sweep's final qubit order by one permutation pass
     void *d_alt = nullptr;          // second state buffer (out-of-place permutation pass)
 

@@ -8,11 +8,14 @@
 #include <algorithm>
 #include <complex>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "subquokka_hip.h"
 #include "engine_internal.hpp"
+#include "host/planner.hpp"
+#include "kernels/fpass.hpp"
 #include "kernels/fused.hpp"
 
 using sq::LoweredGate;
@@ -122,15 +125,17 @@ int launch_apply(sq_engine *e, const cx<double> *u_dev, const std::vector<int> &
 }
 
 // Uploads the program, composes every block in FP64 (one workgroup each) into
-// the matrix area of the fused buffer.  Returns the device matrix array.
-int compose_on_device(sq_engine *e, const FusedProgram &p, int k, cx<double> **u_dev) {
+// the matrix area of the fused buffer.  Returns the device matrix array, and
+// in *extra_dev a 256-byte aligned area of extra bytes after it.
+int compose_on_device(sq_engine *e, const FusedProgram &p, int k, cx<double> **u_dev, size_t extra = 0,
+                      char **extra_dev = nullptr) {
     const size_t nb = p.qsets.size();
     const size_t ops_b = ((p.ops.size() * sizeof(ComposeOp) + 255) / 256) * 256;
     const size_t coef_b = ((p.coef.size() * sizeof(cx<double>) + 255) / 256) * 256;
     const size_t beg_b = ((p.op_begin.size() * sizeof(uint32_t) + 255) / 256) * 256;
     const size_t mat_b = nb * 1024 * sizeof(cx<double>);
     if (hipStreamSynchronize(e->stream) != hipSuccess) return e->fail(SQ_ERR_HIP, "stream sync");
-    int rc = ensure_fused_buffer(e, ops_b + coef_b + beg_b + mat_b + 256);
+    int rc = ensure_fused_buffer(e, ops_b + coef_b + beg_b + mat_b + extra + 256);
     if (rc) return rc;
     char *base = static_cast<char *>(e->d_fused);
     hipError_t he = hipSuccess;
@@ -152,6 +157,7 @@ int compose_on_device(sq_engine *e, const FusedProgram &p, int k, cx<double> **u
     rc = e->check_launch("k_compose");
     if (rc) return rc;
     *u_dev = a.u_out;
+    if (extra_dev) *extra_dev = base + ops_b + coef_b + beg_b + mat_b;
     return SQ_OK;
 }
 
@@ -188,6 +194,180 @@ int build_program(sq_engine *e, const sq_gate *gates, const uint32_t *sizes, uin
         first += sizes[b];
     }
     return SQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fused passes (complex64): the plain gate list cut into passes over tiles of
+// r qubits, each pass's gates grouped into four-qubit blocks (kernels/fpass.hpp).
+// ---------------------------------------------------------------------------
+
+// Value of the tile-fixed qubit set `ext` (bit i = ext[i]) in variant v.
+inline int ext_bit(const std::vector<int> &ext, int q, uint32_t v) {
+    for (size_t i = 0; i < ext.size(); i++) if (ext[i] == q) return (int)((v >> i) & 1u);
+    return -1;
+}
+
+}  // namespace
+
+// One pass of the plan: its tile qubits and its blocks.  A block acts on four
+// tile qubits q (ascending); its diagonal gates may also read the tile-fixed
+// qubits ext, one composed variant per value of them.
+struct FusedPassBlock {
+    std::vector<int> q, ext, gates;
+};
+struct FusedPassStep {
+    std::vector<int> tile;      // r qubits ascending: tile bit i <-> tile[i]
+    std::vector<FusedPassBlock> blocks;
+};
+struct FusedPassPlan {
+    int r = 0;
+    std::vector<FusedPassStep> passes;
+    size_t n_var = 0;           // composed variants over all blocks
+    size_t n_blocks = 0;
+};
+
+namespace {
+
+// Splits the gate list into passes (the register-pass planner with diagonal
+// gates allowed anywhere, r - 4 qubits chosen beside qubits 0-3) and each
+// pass's gates into blocks of at most four tile qubits (the finder's greedy
+// rule, plan_passes with budget 4).  A diagonal gate with qubits outside the
+// tile counts as a gate on its qubits inside (none: a scalar of the tile,
+// attached to the pass's first block).
+int plan_fused_passes(sq_engine *e, const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low, int r,
+                      FusedPassPlan &plan) {
+    const int n = e->n;
+    const size_t ng = gates.size();
+    std::vector<char> mergeable(ng, 1), anywhere(ng, 0);
+    for (size_t i = 0; i < ng; i++) anywhere[i] = low[i].cls == OpClass::Diag;
+    const auto steps = sq::plan_register_passes(gates, mergeable, anywhere, n, 4, r - 4, 0, -1, e->plan_beam);
+    plan = FusedPassPlan{};
+    plan.r = r;
+    for (const auto &st : steps) {
+        FusedPassStep ps;
+        std::vector<char> in(n, 0);
+        for (int q = 0; q < 4; q++) in[q] = 1;
+        for (int q : st.regs) in[q] = 1;
+        int size = 0;
+        for (int q = 0; q < n; q++) size += in[q];
+        for (int q = 0; q < n && size < r; q++) if (!in[q]) { in[q] = 1; size++; }
+        for (int q = 0; q < n; q++) if (in[q]) ps.tile.push_back(q);
+        // proxies: the gates on their tile qubits only
+        std::vector<sq_gate> proxy;
+        std::vector<int> proxy_gate, scalars;
+        for (int gi : st.gates) {
+            const sq_gate &g = gates[gi];
+            std::vector<int> inside;
+            for (int j = 0; j < g.nq; j++) if (in[g.q[j]]) inside.push_back(g.q[j]);
+            if ((int)inside.size() == g.nq) {
+                proxy.push_back(g);
+            } else if (low[gi].cls != OpClass::Diag) {
+                return e->fail(SQ_ERR_ARG, "fused pass: a non-diagonal gate outside the tile");
+            } else if (inside.empty()) {
+                scalars.push_back(gi);
+                continue;
+            } else {
+                sq_gate pg = g;
+                pg.kind = SQ_GATE_P;
+                pg.nq = 1;
+                pg.q[0] = inside[0];
+                proxy.push_back(pg);
+            }
+            proxy_gate.push_back(gi);
+        }
+        auto ext_of = [&](int gi, std::vector<int> &ext) {
+            for (int j = 0; j < gates[gi].nq; j++) {
+                const int q = gates[gi].q[j];
+                if (!in[q] && std::find(ext.begin(), ext.end(), q) == ext.end()) ext.push_back(q);
+            }
+        };
+        auto add_block = [&](const std::vector<int> &gl) {
+            // consecutive pieces of at most kFpMaxExt tile-fixed qubits each
+            size_t i = 0;
+            while (i < gl.size()) {
+                FusedPassBlock b;
+                std::vector<int> used;
+                for (; i < gl.size(); i++) {
+                    std::vector<int> ext = b.ext;
+                    ext_of(gl[i], ext);
+                    if ((int)ext.size() > kFpMaxExt && !b.gates.empty()) break;
+                    b.ext = ext;
+                    b.gates.push_back(gl[i]);
+                    for (int j = 0; j < gates[gl[i]].nq; j++) if (in[gates[gl[i]].q[j]]) used.push_back(gates[gl[i]].q[j]);
+                }
+                std::sort(used.begin(), used.end());
+                used.erase(std::unique(used.begin(), used.end()), used.end());
+                if (used.size() > 4) return false;
+                // padded with the lowest other qubits: among 0-3, always tile qubits
+                b.q = used;
+                for (int q = 0; b.q.size() < 4; q++)
+                    if (std::find(used.begin(), used.end(), q) == used.end()) b.q.push_back(q);
+                std::sort(b.q.begin(), b.q.end());
+                ps.blocks.push_back(std::move(b));
+            }
+            return true;
+        };
+        if (!proxy.empty()) {
+            for (const auto &blk : sq::plan_passes(proxy, n, 4, {})) {
+                std::vector<int> gl;
+                for (int pi : blk.gates) gl.push_back(proxy_gate[pi]);
+                if (!add_block(gl)) return e->fail(SQ_ERR_ARG, "fused pass: block over four qubits");
+            }
+        }
+        if (!scalars.empty()) {
+            // tile scalars commute with every gate of the pass: first block
+            FusedPassBlock &b0 = ps.blocks.empty() ? (ps.blocks.emplace_back(), ps.blocks.back()) : ps.blocks.front();
+            if (b0.q.empty()) b0.q = {0, 1, 2, 3};
+            std::vector<int> rest;
+            for (int gi : scalars) {
+                std::vector<int> ext = b0.ext;
+                ext_of(gi, ext);
+                if ((int)ext.size() > kFpMaxExt) { rest.push_back(gi); continue; }
+                b0.ext = ext;
+                b0.gates.insert(b0.gates.begin(), gi);
+            }
+            if (!rest.empty() && !add_block(rest)) return e->fail(SQ_ERR_ARG, "fused pass: scalar block");
+        }
+        for (const auto &b : ps.blocks) plan.n_var += (size_t)1 << b.ext.size();
+        plan.n_blocks += ps.blocks.size();
+        plan.passes.push_back(std::move(ps));
+    }
+    return SQ_OK;
+}
+
+// Compose ops of one variant of a block (ext bits = v).
+void encode_variant(const FusedPassBlock &b, const std::vector<LoweredGate> &low, uint32_t v, FusedProgram &p) {
+    for (int gi : b.gates) {
+        const LoweredGate &lg = low[gi];
+        int n_ext = 0;
+        for (int j = 0; j < lg.k; j++) n_ext += ext_bit(b.ext, lg.t[j], v) >= 0;
+        if (lg.cls != OpClass::Diag || n_ext == 0) {
+            encode_compose(lg, b.q, p);
+            continue;
+        }
+        // the diagonal restricted to the tile: entries with the fixed bits set
+        LoweredGate d{};
+        d.cls = OpClass::Diag;
+        d.k = 1;
+        if (lg.k == 1 || n_ext == lg.k) {   // scalar of the tile
+            int idx = 0;
+            for (int j = 0; j < lg.k; j++) idx |= ext_bit(b.ext, lg.t[j], v) << j;
+            d.t[0] = b.q[0];
+            d.m[0] = d.m[1] = lg.m[idx];
+        } else {                              // two targets, one inside
+            const int x0 = ext_bit(b.ext, lg.t[0], v), x1 = ext_bit(b.ext, lg.t[1], v);
+            if (x0 >= 0) {      // t[0] fixed, t[1] inside
+                d.t[0] = lg.t[1];
+                d.m[0] = lg.m[x0];
+                d.m[1] = lg.m[x0 + 2];
+            } else {            // t[1] fixed, t[0] inside
+                d.t[0] = lg.t[0];
+                d.m[0] = lg.m[2 * x1];
+                d.m[1] = lg.m[2 * x1 + 1];
+            }
+        }
+        encode_compose(d, b.q, p);
+    }
 }
 
 }  // namespace
@@ -294,6 +474,130 @@ extern "C" int sq_apply_fused(sq_engine *e, const sq_fused *blocks, uint64_t n) 
     for (uint64_t b = 0; b < n; b++) {
         rc = e->dtype == SQ_C128 ? launch_apply<double>(e, u + b * 1024, qsets[b])
                                  : launch_apply<float>(e, u + b * 1024, qsets[b]);
+        if (rc) return rc;
+    }
+    return SQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sq_apply_gates_fused
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int R>
+void launch_fpass(sq_engine *e, const FpArgs &a, unsigned grid) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fpass<R>), grid, 256, 0, e->stream,
+                       reinterpret_cast<cx<float> *>(e->d_state), a);
+}
+
+}  // namespace
+
+extern "C" int sq_apply_gates_fused(sq_engine *e, const sq_gate *gates, uint64_t n_gates, int32_t tile_qbit) {
+    if (!e || (n_gates && !gates)) return SQ_ERR_ARG;
+    if (e->out_of_core()) return e->fail(SQ_ERR_UNSUPPORTED, "fused passes need an HBM-resident state");
+    if (e->dtype != SQ_C64) return e->fail(SQ_ERR_UNSUPPORTED, "fused passes run on complex64 states");
+    if (e->n < kFpMinR) return e->fail(SQ_ERR_ARG, "fused passes need at least 9 qubits");
+    if (tile_qbit != 0 && (tile_qbit < kFpMinR || tile_qbit > kFpMaxR))
+        return e->fail(SQ_ERR_ARG, "tile_qbit must be 0 or 9..13");
+    if (e->n - std::min(e->n, kFpMaxR) > kFpMaxOther) return e->fail(SQ_ERR_ARG, "too many qubits");
+    if (n_gates == 0) return SQ_OK;
+    if (hipSetDevice(e->device) != hipSuccess) return SQ_ERR_HIP;
+    const int r = std::min(e->n, tile_qbit ? tile_qbit : 12);
+    std::vector<sq_gate> gv(gates, gates + n_gates);
+    std::vector<LoweredGate> low(n_gates);
+    for (uint64_t i = 0; i < n_gates; i++) {
+        int rc = e->lower(gv[i], low[i]);
+        if (rc) { e->err += " (gate " + std::to_string(i) + ")"; return rc; }
+    }
+    // plan: cached by structure (qubits and gate classes), as the plain sweep's
+    std::vector<int32_t> key = {e->n, r, e->plan_beam, (int32_t)n_gates};
+    for (uint64_t i = 0; i < n_gates; i++) {
+        key.push_back((int32_t)low[i].cls | (gv[i].nq << 2));
+        for (int j = 0; j < gv[i].nq; j++) key.push_back(gv[i].q[j]);
+    }
+    std::shared_ptr<FusedPassPlan> plan;
+    for (size_t i = 0; i < e->fpass_cache.size(); i++) {
+        if (e->fpass_cache[i].first == key) {
+            plan = e->fpass_cache[i].second;
+            auto ent = std::move(e->fpass_cache[i]);
+            e->fpass_cache.erase(e->fpass_cache.begin() + (long)i);
+            e->fpass_cache.insert(e->fpass_cache.begin(), std::move(ent));
+            break;
+        }
+    }
+    if (!plan) {
+        plan = std::make_shared<FusedPassPlan>();
+        int rc = plan_fused_passes(e, gv, low, r, *plan);
+        if (rc) return rc;
+        e->fpass_cache.insert(e->fpass_cache.begin(), {key, plan});
+        if (e->fpass_cache.size() > 4) e->fpass_cache.pop_back();
+    }
+    // compose programs of every variant, the block records
+    FusedProgram p;
+    p.op_begin.push_back(0);
+    std::vector<FpBlock> recs;
+    uint32_t a_off = 0;
+    for (const auto &ps : plan->passes) {
+        for (const auto &b : ps.blocks) {
+            FpBlock rec{};
+            rec.a_off = a_off;
+            rec.n_ext = (int32_t)b.ext.size();
+            for (size_t i = 0; i < b.ext.size(); i++) rec.ext_q[i] = b.ext[i];
+            for (int j = 0; j < 4; j++)
+                rec.p[j] = (int32_t)(std::lower_bound(ps.tile.begin(), ps.tile.end(), b.q[j]) - ps.tile.begin());
+            recs.push_back(rec);
+            const uint32_t nv = 1u << b.ext.size();
+            for (uint32_t v = 0; v < nv; v++) {
+                encode_variant(b, low, v, p);
+                p.op_begin.push_back((uint32_t)p.ops.size());
+                p.qsets.push_back(b.q);
+            }
+            a_off += nv * kFpAmatU4;
+        }
+    }
+    const size_t n_var = p.qsets.size();
+    const size_t amat_b = ((n_var * kFpAmatU4 * sizeof(u32x4) + 255) / 256) * 256;
+    const size_t rec_b = recs.size() * sizeof(FpBlock);
+    cx<double> *u = nullptr;
+    char *extra = nullptr;
+    int rc = compose_on_device(e, p, 4, &u, amat_b + rec_b, &extra);
+    if (rc) return rc;
+    u32x4 *amat = reinterpret_cast<u32x4 *>(extra);
+    FpBlock *d_recs = reinterpret_cast<FpBlock *>(extra + amat_b);
+    hipError_t he = hipMemcpy(d_recs, recs.data(), rec_b, hipMemcpyHostToDevice);
+    if (he != hipSuccess) return e->hip_fail(he, "hipMemcpy(fused pass blocks)");
+    hipLaunchKernelGGL(k_fpass_amat, (unsigned)((n_var * kFpAmatU4 + 255) / 256), 256, 0, e->stream, u, amat,
+                       (uint32_t)n_var);
+    rc = e->check_launch("k_fpass_amat");
+    if (rc) return rc;
+    const int n = e->n;
+    const unsigned grid = 1u << (n - r);
+    const double bytes = 2.0 * 8.0 * std::ldexp(1.0, n);
+    int b_begin = 0;
+    for (const auto &ps : plan->passes) {
+        FpArgs a{};
+        a.blocks = d_recs;
+        a.amat = amat;
+        a.b_begin = b_begin;
+        a.b_end = b_begin + (int)ps.blocks.size();
+        b_begin = a.b_end;
+        a.r = r;
+        a.n_other = n - r;
+        for (int i = 0; i < r; i++) a.tile_q[i] = ps.tile[i];
+        for (int q = 0, k = 0; q < n; q++)
+            if (!std::binary_search(ps.tile.begin(), ps.tile.end(), q)) a.other_q[k++] = q;
+        // algorithmic flops: the complex 16x16 product, 128 real flops per
+        // amplitude and block (the split products execute three times that)
+        int slot = e->stat_begin("fpass_c64", bytes, 128.0 * (double)ps.blocks.size() * std::ldexp(1.0, n));
+        switch (r) {
+        case 9: launch_fpass<9>(e, a, grid); break;
+        case 10: launch_fpass<10>(e, a, grid); break;
+        case 11: launch_fpass<11>(e, a, grid); break;
+        case 12: launch_fpass<12>(e, a, grid); break;
+        default: launch_fpass<13>(e, a, grid); break;
+        }
+        e->stat_end(slot);
+        rc = e->check_launch("k_fpass");
         if (rc) return rc;
     }
     return SQ_OK;

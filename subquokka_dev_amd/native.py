@@ -77,6 +77,7 @@ _SIGS = {
     "sq_apply_blocks": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64]),
     "sq_apply_fused": (ctypes.c_int, [_P, ctypes.POINTER(SqFused), ctypes.c_uint64]),
     "sq_run_fused": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64]),
+    "sq_apply_gates_fused": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.c_uint64, ctypes.c_int32]),
     "sq_compose_block": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.c_uint64, ctypes.c_int32,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
     "sq_norm2": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
@@ -307,6 +308,12 @@ class Engine:
     def run_fused(self, gates, blocks, n_blocks):
         """Planner blocks (<= 5 qubits each): FP64 composition + MFMA application."""
         self._c(lib().sq_run_fused(self.h, gates, blocks, n_blocks))
+
+    def apply_gates_fused(self, gates, n, tile_qbit=0):
+        """A plain gate list as fused passes (complex64 engines): four-qubit
+        blocks composed in FP64, applied on MFMA inside LDS tiles of
+        tile_qbit qubits (0: the engine's default, 12)."""
+        self._c(lib().sq_apply_gates_fused(self.h, gates, n, tile_qbit))
 
     def compose(self, gates, n, qubits):
         k = len(qubits)

@@ -1,7 +1,8 @@
 // Driver of the CPU emulation build: runs one circuit through the C ABI with
 // the engine compiled for the host (see hip/hip_runtime.h in this folder).
-//   emul <plain|sub|fused|ooc|oocsub|adjoint> <n> <chunk> <c128|c64> <circuit file> [out]
+//   emul <plain|sub|fused|fpass|ooc|oocsub|adjoint> <n> <chunk> <c128|c64> <circuit file> [out]
 // (fused: subcircuit-format file whose blocks act on at most five qubits;
+// fpass: a plain gate list as fused passes, chunk = tile_qbit;
 // ooc / oocsub: the out-of-core runner with resident_qbit = chunk, on a plain
 // gate list / a subcircuit file)
 // Exit code 0 means every launch stayed in bounds (AddressSanitizer aborts
@@ -39,7 +40,7 @@ int main(int argc, char **argv) {
                               sub ? nb : 0, &nb, &line);
     if (rc) { std::fprintf(stderr, "parse rc=%d line %d\n", rc, line); return 3; }
     sq_config cfg{};
-    cfg.total_qbit = n; cfg.chunk_qbit = (ooc || mode == "adjgen") ? 0 : chunk; cfg.dtype = dtype;
+    cfg.total_qbit = n; cfg.chunk_qbit = (ooc || mode == "adjgen" || mode == "fpass") ? 0 : chunk; cfg.dtype = dtype;
     if (ooc) cfg.resident_qbit = chunk;
     if (const char *kh = std::getenv("EMUL_PASS_QBITS")) cfg.pass_qbits = std::atoi(kh);
     if (const char *sm = std::getenv("EMUL_SWEEP_MODE")) cfg.sweep_mode = std::atoi(sm);
@@ -115,6 +116,7 @@ int main(int argc, char **argv) {
         if ((rc = sq_set_state(e, st.data(), 1ULL << n, SQ_C128))) return 9;
     }
     if (mode == "fused") rc = sq_run_fused(e, gates.data(), sizes.data(), nb);
+    else if (mode == "fpass") rc = sq_apply_gates_fused(e, gates.data(), ng, chunk);
     else rc = sub ? sq_apply_blocks(e, gates.data(), sizes.data(), nb) : sq_apply_gates(e, gates.data(), ng);
     if (rc) { std::fprintf(stderr, "apply rc=%d %s\n", rc, sq_last_error(e)); return 5; }
     std::vector<double> out(2ULL << n);
@@ -126,7 +128,9 @@ int main(int argc, char **argv) {
     }
     double nrm = 0;
     if ((rc = sq_norm2(e, &nrm))) return 7;
-    if (!ooc && nrm > 0) {   // measurement shots (addresses only: wave-shared values are not emulated)
+    // measurement shots (addresses only: wave-shared values are not emulated;
+    // after fused passes the state is zero here, the matrix cores being unmodelled)
+    if (!ooc && mode != "fpass" && nrm > 0) {
         std::vector<uint64_t> shots(777);
         if ((rc = sq_sample(e, 5, shots.size(), shots.data()))) { std::fprintf(stderr, "sample rc=%d %s\n", rc, sq_last_error(e)); return 12; }
         for (uint64_t v : shots) if (v >> n) { std::fprintf(stderr, "shot out of range\n"); return 13; }

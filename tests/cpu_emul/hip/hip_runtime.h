@@ -18,6 +18,7 @@
 #define __host__
 #define __shared__
 #define __launch_bounds__(...)
+#define __forceinline__ inline
 #define __align__(x) __attribute__((aligned(x)))
 #define HIP_KERNEL_NAME(...) __VA_ARGS__
 
@@ -100,6 +101,7 @@ inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms =
 // Matrix-core instructions are wave-wide; here they leave the accumulator as is.
 #define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) ((void)(a), (void)(b), (c))
 #define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) ((void)(a), (void)(b), (c))
+#define __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, x, y, z) ((void)(a), (void)(b), (c))
 
 template <typename F, typename... A>
 void sq_emul_launch(F kernel, dim3 grid, dim3 block, size_t, hipStream_t, A... args) {

@@ -115,6 +115,31 @@ def test_fused_small_state_numerics(emul, tmp_path, oracle):
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("n,tile", [(9, 0), (11, 9), (13, 12), (14, 13), (16, 12)])
+def test_fused_pass_launches(emul, tmp_path, n, tile):
+    # sq_apply_gates_fused: the tile loads and stores, every block's LDS
+    # reads and writes, the variant selection and the A-operand reads stay in
+    # bounds (the matrix cores and LDS sharing are not modelled: values are
+    # checked on the GPU).  Every gate kind, so diagonal gates with qubits
+    # outside the tile (variants) and tile scalars occur.
+    text = random_circuit(n, 300, seed=5 * n + tile)
+    run_case(emul, tmp_path, "fpass", n, tile, "c64", text)
+
+
+def test_fused_pass_entry_checks(emul, tmp_path):
+    # complex128 engines and states below 9 qubits are refused
+    text = random_circuit(9, 20, seed=1)
+    f = tmp_path / "c.txt"
+    f.write_text(text)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    for n, dtype in ((9, "c128"), (8, "c64")):
+        t = random_circuit(n, 20, seed=1)
+        f.write_text(t)
+        r = subprocess.run([emul, "fpass", str(n), "0", dtype, str(f)], capture_output=True, text=True, timeout=300,
+                           env=env)
+        assert r.returncode == 5 and "apply rc=" in r.stderr, (r.stdout + r.stderr)[-2000:]
+
+
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("n,layers", [(2, 1), (6, 3), (12, 2)])
 def test_adjoint_launches(emul, tmp_path, dtype, n, layers):
