@@ -161,10 +161,62 @@ def fused_kernel_figures(st, dtype, n):
             "mfma_frac": round(flops / (avg_ms * 1e-3) / 1e12 / peak_tf, 4), "mfma_peak_tflops": peak_tf}
 
 
+def fused_passes_leg(sq, gates, ng, n, args, ref, plain_c128_s):
+    """sq_apply_gates_fused on a complex64 state: the gate list in LDS-resident
+    tile passes, four-qubit blocks on MFMA (split bf16 products, FP32
+    accumulation).  The first call plans (cold_first_call_ms); the timed calls
+    reuse the plan from the engine's cache and compose and apply every block."""
+    eng = sq.Engine(n, dtype="c64")
+    eng.init_zero()
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.apply_gates_fused(gates, ng)
+    eng.sync()
+    cold = time.perf_counter() - t0
+    for _ in range(max(0, args.warmup - 1)):
+        eng.init_zero()
+        eng.apply_gates_fused(gates, ng)
+    eng.sync()
+    eng.stats_reset()
+    eng.stats_enable(True)
+    times = []
+    for _ in range(args.steps):
+        eng.init_zero()
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.apply_gates_fused(gates, ng)
+        eng.sync()
+        times.append(time.perf_counter() - t0)
+        log(f"fused passes c64 step {len(times)}: {times[-1]*1e3:.1f} ms")
+    st = eng.stats()
+    eng.stats_enable(False)
+    k = st["fpass_c64"]
+    avg_ms = k["total_ms"] / k["launches"]
+    bpl = k["alg_bytes"] / k["launches"]
+    # MFMA work as executed: three split products per block (bf16, dense peak 2.5 PFLOP/s)
+    mfma_tf = 3.0 * k["alg_flops"] / (k["total_ms"] * 1e-3) / 1e12
+    ov = ref.inner(eng)
+    infid = 1.0 - abs(ov) ** 2 / (ref.norm2() * eng.norm2())
+    err = ref.max_abs_diff(eng)
+    eng.close()
+    tt = sum(times) / len(times)
+    return {"gates_s": round(ng / tt, 2), "ms_per_step": round(tt * 1e3, 2), "cold_first_call_ms": round(cold * 1e3, 2),
+            "passes": k["launches"] // args.steps, "speedup_vs_plain_c128": round(plain_c128_s / tt, 2),
+            "infidelity_vs_plain_c128": infid, "max_abs_err_vs_plain_c128": err,
+            "kernel": {"name": "k_fpass<13>", "avg_launch_ms": round(avg_ms, 4),
+                       "hbm_gbs": round(bpl / (avg_ms * 1e-3) / 1e9, 1),
+                       "hbm_frac": round(bpl / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "mfma_bf16_tflops_executed": round(mfma_tf, 1),
+                       "mfma_frac": round(mfma_tf / 2500.0, 4), "mfma_peak_tflops": 2500.0},
+            "what": "sq_apply_gates_fused: tile passes of 13 qubits, four-qubit blocks composed in FP64, "
+                    "split-bf16 MFMA products accumulated in FP32"}
+
+
 def workload_fused32(args):
     """32 qubits, 24 layers of the random-layer generator (1152 gates, seed 3):
     planner -> 5-qubit blocks composed in FP64 and applied on MFMA to a
-    complex64 state (32 GiB), against the plain complex128 sweep (64 GiB)."""
+    complex64 state (32 GiB), against the plain complex128 sweep (64 GiB);
+    the same gate list as fused passes (fused_passes_c64) beside it."""
     import subquokka_dev_amd as sq
     n, layers, seed = 32, 24, 3
     text = sq.gen_circuit("random", n, layers, seed=seed)
@@ -213,6 +265,7 @@ def workload_fused32(args):
                kernel=fused_kernel_figures(st, "c64", n),
                compose_ms_per_step=round(st.get("compose_f64", {"total_ms": 0})["total_ms"] / args.steps, 3))
     eng.close()
+    out["fused_passes_c64"] = fused_passes_leg(sq, gates, ng, n, args, ref, t_plain)
     ref.close()
     return out
 
@@ -568,6 +621,7 @@ def main():
                 "plan_s": round(plan_s, 3)}
             eng.close()
         out["fused"] = fused
+        out["fused_passes_c64"] = fused_passes_leg(sq, gates, ng, n, args, eng128, total_t / args.steps)
 
     if not args.no_c64:
         eng64, t64, st64 = run_steps("c64", args.steps, args.warmup, False)

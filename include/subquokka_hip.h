@@ -170,7 +170,7 @@ int sq_apply_fused(sq_engine *e, const sq_fused *blocks, uint64_t n);
 int sq_run_fused(sq_engine *e, const sq_gate *gates, const uint32_t *block_sizes, uint64_t n_blocks);
 /* A plain gate list (the sq_apply_gates / MEM_Runner::run(vector<Gate*>&) input,
  * MEM_Runner.cpp:101-134) as fused passes on a complex64 state: the engine cuts
- * the list into passes over tiles of tile_qbit qubits (0: 12; else 9..13, at
+ * the list into passes over tiles of tile_qbit qubits (0: 13; else 9..13, at
  * most N), groups each pass's gates into four-qubit blocks, composes each block
  * in FP64 on the GPU and applies it to the tile in LDS on MFMA with split bf16
  * products accumulated in FP32 (about 1e-5 relative per block; the reference's

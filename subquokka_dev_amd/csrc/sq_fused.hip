@@ -335,6 +335,62 @@ int plan_fused_passes(sq_engine *e, const std::vector<sq_gate> &gates, const std
     return SQ_OK;
 }
 
+// LDS offsets of a block (kernels/fpass.hpp): the block's tile bits p, four
+// column bits and the remaining batch bits.  Column bits are taken lowest
+// first among the non-block tile bits whose bank-pair vectors (s(1 << q) & 31)
+// keep the column span independent and free of the vectors of block bits 1
+// and 2; failing that, independent only; failing that, any.
+void fill_offsets(const std::vector<int> &tile, const std::vector<int> &q, int r, FpBlock &rec) {
+    int p[4];
+    for (int j = 0; j < 4; j++) p[j] = (int)(std::lower_bound(tile.begin(), tile.end(), q[j]) - tile.begin());
+    std::vector<int> nb;
+    for (int i = 0; i < r; i++) if (i != p[0] && i != p[1] && i != p[2] && i != p[3]) nb.push_back(i);
+    auto vec = [](int i) { return fp_slot(1u << i) & 31u; };
+    // GF(2) span of a vector set kept as a basis with distinct leading bits
+    auto reduce = [](uint32_t v, const std::vector<uint32_t> &basis) {
+        for (uint32_t b : basis) if ((v ^ b) < v) v ^= b;
+        return v;
+    };
+    auto add = [&](std::vector<uint32_t> &basis, uint32_t v) {
+        basis.push_back(reduce(v, basis));
+        std::sort(basis.begin(), basis.end(), [](uint32_t x, uint32_t y) { return x > y; });
+    };
+    std::vector<int> col;
+    for (int strict = 2; strict >= 0 && col.size() < 4; strict--) {
+        col.clear();
+        std::vector<uint32_t> basis;
+        for (int i : nb) {
+            if (col.size() == 4) break;
+            const uint32_t v = reduce(vec(i), basis);
+            if (strict >= 1 && !v) continue;
+            if (strict == 2) {
+                std::vector<uint32_t> b2 = basis;
+                add(b2, vec(i));
+                if (!reduce(vec(p[1]), b2) || !reduce(vec(p[2]), b2)) continue;
+            }
+            col.push_back(i);
+            if (v) add(basis, vec(i));
+        }
+    }
+    std::vector<int> rest;
+    for (int i : nb) if (std::find(col.begin(), col.end(), i) == col.end()) rest.push_back(i);
+    auto deposit = [](uint32_t x, const std::vector<int> &pos) {
+        uint32_t o = 0;
+        for (size_t k = 0; k < pos.size(); k++) o |= ((x >> k) & 1u) << pos[k];
+        return o;
+    };
+    const std::vector<int> pv(p, p + 4);
+    for (uint32_t i = 0; i < 16; i++) {
+        rec.colw[i] = fp_slot(deposit(i, col)) * 8u;
+        rec.blkw[i] = fp_slot(deposit(i, pv)) * 8u;
+    }
+    const uint32_t nw = (uint32_t)fp_waves(r);
+    const uint32_t batches = 1u << (r - 8), per_wave = (batches + nw - 1) / nw;
+    for (uint32_t w = 0; w < nw; w++)
+        for (uint32_t i = 0; i < per_wave; i++)
+            if (w + nw * i < batches) rec.ubw[w * per_wave + i] = fp_slot(deposit(w + nw * i, rest)) * 8u;
+}
+
 // Compose ops of one variant of a block (ext bits = v).
 void encode_variant(const FusedPassBlock &b, const std::vector<LoweredGate> &low, uint32_t v, FusedProgram &p) {
     for (int gi : b.gates) {
@@ -486,7 +542,7 @@ namespace {
 
 template <int R>
 void launch_fpass(sq_engine *e, const FpArgs &a, unsigned grid) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fpass<R>), grid, 256, 0, e->stream,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fpass<R>), grid, 64 * fp_waves(R), 0, e->stream,
                        reinterpret_cast<cx<float> *>(e->d_state), a);
 }
 
@@ -502,7 +558,7 @@ extern "C" int sq_apply_gates_fused(sq_engine *e, const sq_gate *gates, uint64_t
     if (e->n - std::min(e->n, kFpMaxR) > kFpMaxOther) return e->fail(SQ_ERR_ARG, "too many qubits");
     if (n_gates == 0) return SQ_OK;
     if (hipSetDevice(e->device) != hipSuccess) return SQ_ERR_HIP;
-    const int r = std::min(e->n, tile_qbit ? tile_qbit : 12);
+    const int r = std::min(e->n, tile_qbit ? tile_qbit : 13);
     std::vector<sq_gate> gv(gates, gates + n_gates);
     std::vector<LoweredGate> low(n_gates);
     for (uint64_t i = 0; i < n_gates; i++) {
@@ -543,8 +599,7 @@ extern "C" int sq_apply_gates_fused(sq_engine *e, const sq_gate *gates, uint64_t
             rec.a_off = a_off;
             rec.n_ext = (int32_t)b.ext.size();
             for (size_t i = 0; i < b.ext.size(); i++) rec.ext_q[i] = b.ext[i];
-            for (int j = 0; j < 4; j++)
-                rec.p[j] = (int32_t)(std::lower_bound(ps.tile.begin(), ps.tile.end(), b.q[j]) - ps.tile.begin());
+            fill_offsets(ps.tile, b.q, r, rec);
             recs.push_back(rec);
             const uint32_t nv = 1u << b.ext.size();
             for (uint32_t v = 0; v < nv; v++) {
@@ -581,7 +636,6 @@ extern "C" int sq_apply_gates_fused(sq_engine *e, const sq_gate *gates, uint64_t
         a.b_begin = b_begin;
         a.b_end = b_begin + (int)ps.blocks.size();
         b_begin = a.b_end;
-        a.r = r;
         a.n_other = n - r;
         for (int i = 0; i < r; i++) a.tile_q[i] = ps.tile[i];
         for (int q = 0, k = 0; q < n; q++)

@@ -312,7 +312,7 @@ class Engine:
     def apply_gates_fused(self, gates, n, tile_qbit=0):
         """A plain gate list as fused passes (complex64 engines): four-qubit
         blocks composed in FP64, applied on MFMA inside LDS tiles of
-        tile_qbit qubits (0: the engine's default, 12)."""
+        tile_qbit qubits (0: the engine's default, 13)."""
         self._c(lib().sq_apply_gates_fused(self.h, gates, n, tile_qbit))
 
     def compose(self, gates, n, qubits):
