@@ -101,7 +101,7 @@ typedef struct sq_config {
                                  amplitudes per streamed chunk (DIO_Runner role) */
     int32_t sweep_mode;       /* 0 = register passes (a run of one-qubit and diagonal
                                  gates per read/write of the state); subcircuits with
-                                 VSWAPs relabelled and one set of qubit swaps at the end.
+                                 VSWAPs relabelled and one permutation pass at the end.
                                  1 = one launch per gate; subcircuits as VSWAP sweeps
                                  and one LDS tile pass per gate block */
     int32_t pass_qbits;       /* register qubits per pass, 1..5 (0: 5) */

@@ -143,6 +143,7 @@ void Simulator::setup_ini(const std::string &ini) {
     env_.state_paths = reader.get(s, "state_paths", "");
     env_.precision = reader.get(s, "precision", "c128");
     env_.fused_k = (int)reader.get_int(s, "fused_k", 0);
+    env_.fused_passes = (int)reader.get_int(s, "fused_passes", 0);
     env_.resident_qbit = (int)reader.get_int(s, "resident_qbit", 0);
     env_.init_seed = (uint64_t)reader.get_int(s, "init_seed", 0);
     env_.dump_binary = reader.get(s, "dump_binary", "");
@@ -151,6 +152,10 @@ void Simulator::setup_ini(const std::string &ini) {
         throw SimError(SQ_ERR_ARG, "[Config File]: precision must be c128 or c64");
     if (env_.fused_k < 0 || env_.fused_k > 5)
         throw SimError(SQ_ERR_ARG, "[Config File]: fused_k must be 0 (off) or 1..5");
+    if (env_.fused_passes != 0 && env_.fused_passes != 1)
+        throw SimError(SQ_ERR_ARG, "[Config File]: fused_passes must be 0 or 1");
+    if (env_.fused_passes && (env_.precision != "c64" || env_.fused_k))
+        throw SimError(SQ_ERR_ARG, "[Config File]: fused_passes needs precision = c64 and fused_k = 0");
     if (seg_.mpi > 0)
         throw SimError(SQ_ERR_UNSUPPORTED, "[Config File]: mpi_qbit > 0 needs several GPUs; this engine runs on one");
 
@@ -274,9 +279,15 @@ void HBM_Runner::check(int rc, const char *what) {
 
 // MEM_Runner::run(vector<Gate*>&) (MEM_Runner.cpp:101-134).  With fused_k > 0
 // (state in HBM) the planner groups the gates into blocks of at most fused_k
-// qubits, each composed in FP64 and applied in one MFMA pass; the planning is
-// inside the timed region, like every other host step of the run.
+// qubits, each composed in FP64 and applied in one MFMA pass; with
+// fused_passes = 1 (complex64) the engine runs the list as fused passes
+// (four-qubit blocks in LDS-resident tiles).  The planning is inside the timed
+// region, like every other host step of the run.
 void HBM_Runner::run(std::vector<sq_gate> &circuit) {
+    if (env_.fused_passes && !streamed_) {
+        check(sq_apply_gates_fused(e_, circuit.data(), circuit.size(), 0), "sq_apply_gates_fused");
+        return;
+    }
     if (env_.fused_k > 0 && !streamed_) {
         std::vector<sq_gate> order;
         std::vector<uint32_t> sizes;

@@ -62,6 +62,7 @@ struct Env {
     // engine keys (defaults keep the reference's behaviour)
     std::string precision = "c128";   // c128 | c64
     int fused_k = 0;                   // >0: plan the plain circuit into fused k-qubit blocks
+    int fused_passes = 0;              // 1: the plain circuit as fused passes (complex64)
     int resident_qbit = 0;             // Stream_Runner chunk width (0 = min(N, 30))
     uint64_t init_seed = 0;            // 0: |0...0> (reference); >0: seeded Gaussian state
     std::string dump_binary;           // raw little-endian (re, im) doubles of the final state

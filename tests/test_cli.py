@@ -47,6 +47,8 @@ def test_missing_ini(tmp_path):
     (dict(total_qbit=6, chunk_qbit=2, mpi_qbit=1, runner_type="MPI_MEM"), "mpi_qbit"),
     (dict(total_qbit=6, chunk_qbit=2, runner_type="MEM", precision="c32"), "precision"),
     (dict(total_qbit=6, chunk_qbit=2, runner_type="MEM", fused_k=7), "fused_k"),
+    (dict(total_qbit=10, chunk_qbit=2, runner_type="MEM", fused_passes=1), "fused_passes"),
+    (dict(total_qbit=10, chunk_qbit=2, runner_type="MEM", precision="c64", fused_passes=2), "fused_passes"),
     (dict(total_qbit=8, chunk_qbit=2, file_qbit=2, runner_type="DirectIO", state_paths="a,b"), "state_paths"),
 ])
 def test_config_errors(tmp_path, kv, msg):
@@ -181,6 +183,21 @@ def test_cli_fused_k_blocks(tmp_path, oracle, precision, tol):
     assert r.returncode == 0, r.stderr
     want = oracle.run(sq.gen_state(n, 4), n, text)
     assert np.max(np.abs(dumped(tmp_path / "out.bin", n) - want)) < tol
+
+
+@pytest.mark.gpu
+def test_cli_fused_passes(tmp_path, oracle):
+    # fused_passes=1 (complex64): the plain circuit as fused passes
+    n = 14
+    text = random_circuit(n, 300, seed=13)
+    open(tmp_path / "c.txt", "w").write(text)
+    write_ini(tmp_path / "a.ini", total_qbit=n, chunk_qbit=4, runner_type="MEM", is_subcircuit=0,
+              precision="c64", fused_passes=1, init_seed=5, dump_binary="out.bin")
+    r = run(["-i", "a.ini", "-c", "c.txt"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    want = oracle.run(sq.gen_state(n, 5), n, text)
+    got = dumped(tmp_path / "out.bin", n)
+    assert np.max(np.abs(got - want)) <= 1e-4 * np.max(np.abs(want))
 
 
 @pytest.mark.gpu
