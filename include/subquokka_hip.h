@@ -250,6 +250,16 @@ typedef struct sq_kstat {
 int sq_stats_enable(sq_engine *e, int32_t on);
 int sq_stats_get(sq_engine *e, sq_kstat *out, int32_t cap, int32_t *n_out);
 int sq_stats_reset(sq_engine *e);
+/* Op census of the engine's last register-pass program (diagnostics; no
+ * reference counterpart): how many ops of each form its passes run.
+ *   0 dense on a register slot, 1 dense on lane bits 0-3, 2 dense on lane bits
+ *   4-5, 3 dense inside the complex64 vector, 4 diagonal without register
+ *   bits, 5 diagonal with one register bit, 6 diagonal with two, 7 unused;
+ *   8 + 4 t + c: a controlled phase folded into a dense op (complex128), target
+ *   t = 0 register slot, 1 lane bits 0-3, 2 lane bits 4-5; control c = 0
+ *   register slot 0, 1 another register slot, 2 a lane bit, 3 a tile bit.
+ * *n_out = 20; SQ_ERR_ARG when cap < 20 (the first cap counts are written). */
+int sq_pass_census(sq_engine *e, int64_t *counts, int32_t cap, int32_t *n_out);
 /* Out-of-core runner telemetry (no reference counterpart; DIO_Runner.cpp has
  * only the "<t>s" line): streamed passes since the last sq_stats_reset, the
  * bytes those passes moved over the host link (both directions) and the host

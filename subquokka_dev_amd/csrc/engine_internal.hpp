@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <atomic>
 #include <cstdint>
 #include <memory>
@@ -85,6 +86,7 @@ struct sq_engine {
     std::vector<PlanCacheEntry> plan_cache;
     // fused-pass plans of recent gate lists, keyed the same way, most recent first
     std::vector<std::pair<std::vector<int32_t>, std::shared_ptr<FusedPassPlan>>> fpass_cache;
+    std::array<int64_t, 20> pass_census{};   // op kinds of the last register-pass program (sq_pass_census)
     bool bitperm_ok = true;         // the blocked sweep's final qubit order by one permutation pass
     void *d_alt = nullptr;          // second state buffer (out-of-place permutation pass)
 

@@ -286,6 +286,29 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
         prog.launches.push_back(l);
     }
 
+    // op census of this program (sq_pass_census)
+    e->pass_census.fill(0);
+    for (const auto &o : ops) {
+        int idx = -1;
+        switch (o.kind) {
+        case PO_DENSE_REG: idx = 0; break;
+        case PO_DENSE_LANE: idx = o.a < 4 ? 1 : 2; break;
+        case PO_DENSE_VEC: idx = 3; break;
+        case PO_DIAG_FREE: idx = 4; break;
+        case PO_DIAG_REG1: idx = 5; break;
+        case PO_DIAG_REG2: idx = 6; break;
+        case PO_DENSE_REG_C:
+        case PO_DENSE_LANE_C: {
+            const int t = o.kind == PO_DENSE_REG_C ? 0 : (o.a < 4 ? 1 : 2);
+            const int c = o.active == 0 ? (o.b == 0 ? 0 : 1) : (o.active == 1 ? 2 : 3);
+            idx = 8 + 4 * t + c;
+            break;
+        }
+        default: break;
+        }
+        if (idx >= 0) e->pass_census[idx]++;
+    }
+
     // upload the program once; earlier launches may still read the old one
     const size_t need = ops.size() * sizeof(PassOp<T>) + 256;
     if (dst) {   // caller-owned buffer: never grown
@@ -468,4 +491,12 @@ extern "C" int sq_apply_gates_noalloc(sq_engine *e, const sq_gate *gates, uint64
     PlainProgram prog;
     int rc = e->build_plain(gl, low, prog, e->d_scratch, kScratchBytes);
     return rc ? rc : e->run_plain_program(prog, low);
+}
+
+extern "C" int sq_pass_census(sq_engine *e, int64_t *counts, int32_t cap, int32_t *n_out) {
+    if (!e || (cap > 0 && !counts)) return SQ_ERR_ARG;
+    const int32_t n = (int32_t)e->pass_census.size();
+    if (n_out) *n_out = n;
+    for (int32_t i = 0; i < std::min(cap, n); i++) counts[i] = e->pass_census[i];
+    return cap < n ? SQ_ERR_ARG : SQ_OK;
 }

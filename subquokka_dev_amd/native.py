@@ -78,6 +78,8 @@ _SIGS = {
     "sq_apply_fused": (ctypes.c_int, [_P, ctypes.POINTER(SqFused), ctypes.c_uint64]),
     "sq_run_fused": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64]),
     "sq_apply_gates_fused": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.c_uint64, ctypes.c_int32]),
+    "sq_pass_census": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32,
+                                      ctypes.POINTER(ctypes.c_int32)]),
     "sq_compose_block": (ctypes.c_int, [_P, ctypes.POINTER(SqGate), ctypes.c_uint64, ctypes.c_int32,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
     "sq_norm2": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
@@ -314,6 +316,21 @@ class Engine:
         blocks composed in FP64, applied on MFMA inside LDS tiles of
         tile_qbit qubits (0: the engine's default, 13)."""
         self._c(lib().sq_apply_gates_fused(self.h, gates, n, tile_qbit))
+
+    def pass_census(self):
+        """Op counts of the last register-pass program by form (sq_pass_census):
+        a dict with 'dense_reg', 'dense_lane03', 'dense_lane45', 'dense_vec',
+        'diag_free', 'diag_reg1', 'diag_reg2' and 'fold' -- a 3x4 array of folded
+        controlled phases by target (register, lane 0-3, lane 4-5) and control
+        (register slot 0, other slot, lane, tile)."""
+        c = (ctypes.c_int64 * 20)()
+        n = ctypes.c_int32()
+        self._c(lib().sq_pass_census(self.h, c, 20, ctypes.byref(n)))
+        v = list(c)
+        names = ["dense_reg", "dense_lane03", "dense_lane45", "dense_vec", "diag_free", "diag_reg1", "diag_reg2"]
+        out = {k: v[i] for i, k in enumerate(names)}
+        out["fold"] = np.array(v[8:20], dtype=np.int64).reshape(3, 4)
+        return out
 
     def compose(self, gates, n, qubits):
         k = len(qubits)

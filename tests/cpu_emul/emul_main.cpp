@@ -119,6 +119,14 @@ int main(int argc, char **argv) {
     else if (mode == "fpass") rc = sq_apply_gates_fused(e, gates.data(), ng, chunk);
     else rc = sub ? sq_apply_blocks(e, gates.data(), sizes.data(), nb) : sq_apply_gates(e, gates.data(), ng);
     if (rc) { std::fprintf(stderr, "apply rc=%d %s\n", rc, sq_last_error(e)); return 5; }
+    if (std::getenv("EMUL_CENSUS")) {   // op census of the last register-pass program
+        int64_t cnt[20];
+        int32_t nc = 0;
+        if (sq_pass_census(e, cnt, 20, &nc)) return 15;
+        std::printf("census");
+        for (int i = 0; i < nc; i++) std::printf(" %lld", (long long)cnt[i]);
+        std::printf("\n");
+    }
     std::vector<double> out(2ULL << n);
     if ((rc = sq_get_state(e, out.data(), 1ULL << n, SQ_C128))) { std::fprintf(stderr, "get rc=%d\n", rc); return 6; }
     if (argc == 7) {   // optional state dump (interleaved doubles)

@@ -131,6 +131,20 @@ def gate_line(kind, qs, rng_py, nrng):
     return f"{kind} {a} {b} {rng_py.uniform(0, 2 * math.pi)!r}"
 
 
+def fold_pairs_circuit(n, n_pairs, seed):
+    """CP on a random pair followed by a dense gate on one of its qubits: the
+    complex128 register passes fold most of these phases into the dense
+    gate's op, in every target/control placement at n = 14."""
+    rng = random.Random(seed)
+    nrng = np.random.default_rng(seed)
+    lines = []
+    for _ in range(n_pairs):
+        a, b = rng.sample(range(n), 2)
+        lines.append(f"CP {min(a, b)} {max(a, b)} {rng.uniform(0, 6.28)!r}")
+        lines.append(gate_line(rng.choice(["U1", "H", "RX"]), [rng.choice([a, b])], rng, nrng))
+    return "\n".join(lines) + "\n"
+
+
 def random_circuit(n, n_gates, seed, kinds=ALL_KINDS, sorted_2q=True):
     rng = random.Random(seed)
     nrng = np.random.default_rng(seed)

@@ -115,6 +115,28 @@ def test_fused_small_state_numerics(emul, tmp_path, oracle):
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
 
 
+def test_fold_census_covers_every_form(emul, tmp_path):
+    # The op census of the complex128 register-pass program (host side, so the
+    # emulation computes it exactly, with the engine's default three lane
+    # moves): the circuit the GPU fold test runs holds every folded form --
+    # targets on register slots, lane bits 0-3 (DPP) and 4-5 (permlane swap),
+    # controls in register slot 0 (swapped into lane bit 4/5 by the permlane
+    # path), another slot, a lane bit and a tile bit.
+    from helpers import fold_pairs_circuit
+    n = 14
+    f = tmp_path / "c.txt"
+    f.write_text(fold_pairs_circuit(n, 400, seed=0))
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", EMUL_CENSUS="1", EMUL_LANE_MOVES="3")
+    r = subprocess.run([emul, "plain", str(n), str(n), "c128", str(f)], capture_output=True, text=True, timeout=600,
+                       env=env)
+    assert r.returncode == 0, (r.stdout + r.stderr)[-2000:]
+    census = [int(x) for x in next(l for l in r.stdout.splitlines() if l.startswith("census")).split()[1:]]
+    assert len(census) == 20
+    fold = np.array(census[8:20]).reshape(3, 4)
+    assert (fold > 0).all(), fold
+    assert census[0] > 0 and census[1] > 0 and census[2] > 0 and census[4] > 0 and census[5] > 0
+
+
 @pytest.mark.parametrize("n,tile", [(9, 0), (11, 9), (13, 12), (14, 13), (16, 12)])
 def test_fused_pass_launches(emul, tmp_path, n, tile):
     # sq_apply_gates_fused: the tile loads and stores, every block's LDS

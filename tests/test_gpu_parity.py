@@ -205,6 +205,30 @@ def test_register_passes_fold_controlled_phases(oracle, kh):
         b_.close()
 
 
+def test_register_passes_every_fold_form(oracle):
+    # every folded form present in the program (census) and the result equal to
+    # one launch per gate and to the oracle
+    from helpers import fold_pairs_circuit
+    n = 14
+    text = fold_pairs_circuit(n, 400, seed=0)
+    s0 = random_state(n, 7)
+    say("register passes: every fold form")
+    a_ = sq.Engine(n, dtype="c128")
+    b_ = sq.Engine(n, dtype="c128", sweep_mode=1)
+    try:
+        a_.set_state(s0)
+        b_.set_state(s0)
+        a_.apply_text(text)
+        b_.apply_text(text)
+        fold = a_.pass_census()["fold"]
+        assert (fold > 0).all(), fold
+        assert a_.max_abs_diff(b_) <= 1e-14
+        check(a_.get_state(np.complex128), oracle.run(s0, n, text), "c128", "every fold form")
+    finally:
+        a_.close()
+        b_.close()
+
+
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("n", [10, 11, 12])
 def test_register_passes_smallest_states(oracle, dtype, n):
