@@ -138,6 +138,10 @@ struct sq_engine {
     int lower(const sq_gate &g, sq::LoweredGate &lg);
     int apply_lowered(const sq::LoweredGate &g);
     int apply_plain(const std::vector<sq_gate> &gates, const std::vector<sq::LoweredGate> &low);
+    // CNOTs (exact U2 permutation matrices) as H CZ H when register passes run;
+    // false (g2, l2 untouched) when nothing changes
+    bool expand_cnots(const std::vector<sq_gate> &gates, const std::vector<sq::LoweredGate> &low,
+                      std::vector<sq_gate> &g2, std::vector<sq::LoweredGate> &l2) const;
     // dst: where the op records go (nullptr: the engine's own growable buffer;
     // otherwise a caller-owned device buffer of dst_cap bytes, never grown)
     int build_plain(const std::vector<sq_gate> &gates, const std::vector<sq::LoweredGate> &low, PlainProgram &prog,

@@ -118,14 +118,31 @@ inline int lower_gate(const sq_gate &g, int n_qubits, LoweredGate &out, std::str
         out.cls = OpClass::Dense; out.name = "U1";
         for (int i = 0; i < 4; i++) out.m[i] = C(g.mat[2 * i], g.mat[2 * i + 1]);
         break;
-    case SQ_GATE_U2:  // circuit.cpp:2811-2819
+    case SQ_GATE_U2: {  // circuit.cpp:2811-2819
         out.cls = OpClass::Dense; out.name = "U2";
         for (int i = 0; i < 16; i++) {
             int b1 = i & 1, b2 = (i >> 1) & 1, b3 = (i >> 2) & 1, b4 = (i >> 3) & 1;
             int res = (b3 << 3) | (b4 << 2) | (b1 << 1) | b2;
             out.m[i] = C(g.mat[2 * res], g.mat[2 * res + 1]);
         }
-        break;
+        // An exactly diagonal matrix (a CZ, a controlled phase written as U2)
+        // is a diagonal gate: the same products, only the amplitudes it changes
+        // are visited, and register passes carry it on any qubits.
+        bool diagonal = true;
+        uint32_t active = 0;
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++)
+                if (r != c && out.m[r * 4 + c] != zero) diagonal = false;
+        for (int r = 0; r < 4; r++) if (out.m[r * 5] != one) active |= 1u << r;
+        if (diagonal && active) {
+            const C d[4] = {out.m[0], out.m[5], out.m[10], out.m[15]};
+            out.cls = OpClass::Diag;
+            for (int r = 0; r < 4; r++) out.m[r] = d[r];
+            for (int i = 4; i < 16; i++) out.m[i] = zero;
+            out.diag_active = active;
+            out.pass_fraction = __builtin_popcount(active) / 4.0;
+        }
+        break; }
     case SQ_GATE_SWAP: case SQ_GATE_VSWAP_1_1:  // SWAPGate circuit.cpp:351-355
         out.cls = OpClass::Perm; out.name = g.kind == SQ_GATE_SWAP ? "SWAP" : "VSWAP_1_1";
         out.n_pairs = 1; out.pass_fraction = 0.5;

@@ -433,10 +433,88 @@ int sq_engine::run_plain_program(const PlainProgram &prog, const std::vector<Low
     return dtype == SQ_C128 ? run_plain_t<double>(this, prog, low) : run_plain_t<float>(this, prog, low);
 }
 
+namespace {
+
+// The target of a U2 whose matrix is exactly a CNOT (control either qubit),
+// or -1.  Local index bit 0 is t[0], bit 1 is t[1].
+int cnot_target(const LoweredGate &g) {
+    if (g.cls != OpClass::Dense || g.k != 2) return -1;
+    const std::complex<double> one(1.0, 0.0), zero(0.0, 0.0);
+    auto is = [&](const int *perm) {   // row r has its one at column perm[r]
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++)
+                if (g.m[r * 4 + c] != (c == perm[r] ? one : zero)) return false;
+        return true;
+    };
+    static const int flip1[4] = {0, 3, 2, 1};   // control t[0]: bit 1 flips where bit 0 is set
+    static const int flip0[4] = {0, 1, 3, 2};   // control t[1]: bit 0 flips where bit 1 is set
+    if (is(flip1)) return g.t[1];
+    if (is(flip0)) return g.t[0];
+    return -1;
+}
+
+}  // namespace
+
+// Register passes carry one-qubit dense and diagonal gates; a CNOT (written
+// as U2, the reference's way to give one) is H_t CZ H_t with exact 1/sqrt(2)
+// coefficients, so it joins a pass (on complex128 the CZ folds into the
+// second H) instead of taking a pass of its own.  The products differ from
+// the dense 4x4 application by rounding only (|H H - I| = 2^-52).
+bool sq_engine::expand_cnots(const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low,
+                             std::vector<sq_gate> &g2, std::vector<LoweredGate> &l2) const {
+    const int S = (dtype == SQ_C64 ? 1 : 0) + 6;
+    if (sweep_mode != 0 || n < S + pass_qbits) return false;   // no register passes
+    bool any = false;
+    for (const auto &g : low) any = any || cnot_target(g) >= 0;
+    if (!any) return false;
+    const double s = 0.70710678118654752440;
+    g2.clear();
+    l2.clear();
+    for (size_t i = 0; i < low.size(); i++) {
+        const int t = cnot_target(low[i]);
+        if (t < 0) {
+            g2.push_back(gates[i]);
+            l2.push_back(low[i]);
+            continue;
+        }
+        LoweredGate h{};
+        h.cls = OpClass::Dense;
+        h.k = 1;
+        h.t[0] = t;
+        h.m[0] = h.m[1] = h.m[2] = s;
+        h.m[3] = -s;
+        h.pass_fraction = 1.0;
+        h.name = "H (CNOT)";
+        LoweredGate cz{};
+        cz.cls = OpClass::Diag;
+        cz.k = 2;
+        cz.t[0] = low[i].t[0];
+        cz.t[1] = low[i].t[1];
+        cz.m[0] = cz.m[1] = cz.m[2] = 1.0;
+        cz.m[3] = -1.0;
+        cz.diag_active = 8u;
+        cz.pass_fraction = 0.25;
+        cz.name = "CZ (CNOT)";
+        sq_gate gh = gates[i], gz = gates[i];
+        gh.kind = SQ_GATE_U1;
+        gh.nq = 1;
+        gh.q[0] = t;
+        gz.kind = SQ_GATE_CP;
+        gz.q[0] = cz.t[0];
+        gz.q[1] = cz.t[1];
+        g2.insert(g2.end(), {gh, gz, gh});
+        l2.insert(l2.end(), {h, cz, h});
+    }
+    return true;
+}
+
 int sq_engine::apply_plain(const std::vector<sq_gate> &gates, const std::vector<LoweredGate> &low) {
+    std::vector<sq_gate> g2;
+    std::vector<LoweredGate> l2;
+    const bool x = expand_cnots(gates, low, g2, l2);
     PlainProgram prog;
-    int rc = build_plain(gates, low, prog);
-    return rc ? rc : run_plain_program(prog, low);
+    int rc = build_plain(x ? g2 : gates, x ? l2 : low, prog);
+    return rc ? rc : run_plain_program(prog, x ? l2 : low);
 }
 
 // ---------------------------------------------------------------------------
@@ -488,9 +566,12 @@ extern "C" int sq_apply_gates_noalloc(sq_engine *e, const sq_gate *gates, uint64
         if (rc) { e->err += " (gate " + std::to_string(i) + ")"; return rc; }
     }
     const std::vector<sq_gate> gl(gates, gates + n);
+    std::vector<sq_gate> g2;
+    std::vector<LoweredGate> l2;
+    const bool x = e->expand_cnots(gl, low, g2, l2);
     PlainProgram prog;
-    int rc = e->build_plain(gl, low, prog, e->d_scratch, kScratchBytes);
-    return rc ? rc : e->run_plain_program(prog, low);
+    int rc = e->build_plain(x ? g2 : gl, x ? l2 : low, prog, e->d_scratch, kScratchBytes);
+    return rc ? rc : e->run_plain_program(prog, x ? l2 : low);
 }
 
 extern "C" int sq_pass_census(sq_engine *e, int64_t *counts, int32_t cap, int32_t *n_out) {

@@ -115,6 +115,41 @@ def test_fused_small_state_numerics(emul, tmp_path, oracle):
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_cnot_and_diagonal_u2_in_register_passes(emul, tmp_path, oracle, dtype):
+    # CNOTs written as U2 run as H CZ H inside register passes, diagonal U2 as
+    # diagonal ops.  Every gate acts on qubits 7-12 (register and tile bits,
+    # no lane exchange), so the emulation computes the values exactly: they
+    # must equal the oracle's dense 4x4 application.
+    import random as _random
+    from helpers import gate_line
+    from test_gpu_parity import cnot_line
+    n = 13
+    rng = _random.Random(5)
+    nrng = np.random.default_rng(5)
+    lines = []
+    for _ in range(160):
+        a, b = rng.sample(range(7, n), 2)
+        r = rng.random()
+        if r < 0.4:
+            lines.append(cnot_line(a, b))
+        elif r < 0.5:
+            ph = rng.uniform(0, 6.28)
+            d = [1, 1, 1, complex(np.cos(ph), np.sin(ph))]
+            m = []
+            for i in range(4):
+                for j in range(4):
+                    v = complex(d[i] if i == j else 0)
+                    m += [v.real, v.imag]
+            lines.append(f"U2 {min(a, b)} {max(a, b)} " + " ".join(repr(float(x)) for x in m))
+        else:
+            lines.append(gate_line(rng.choice(["U1", "H", "RX"]), [a], rng, nrng))
+    text = "\n".join(lines) + "\n"
+    got = run_case(emul, tmp_path, "plain", n, n, dtype, text, seed=9)
+    want = oracle.run(sq.gen_state(n, 9), n, text)
+    assert np.max(np.abs(got - want)) <= (1e-12 if dtype == "c128" else 2e-6)
+
+
 def test_fold_census_covers_every_form(emul, tmp_path):
     # The op census of the complex128 register-pass program (host side, so the
     # emulation computes it exactly, with the engine's default three lane

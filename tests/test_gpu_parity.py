@@ -205,6 +205,66 @@ def test_register_passes_fold_controlled_phases(oracle, kh):
         b_.close()
 
 
+def cnot_line(c, t):
+    # CNOT c -> t in the reference's U2 layout (basis |first second>, first the
+    # most significant; targets ascending)
+    ct = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0]   # control is the first (smaller) qubit
+    tc = [1, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0]   # control is the second
+    m = ct if c < t else tc
+    return f"U2 {min(c, t)} {max(c, t)} " + " ".join(f"{v} 0" for v in m)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_register_passes_cnot_and_diagonal_u2(oracle, dtype):
+    # CNOTs written as U2 run inside register passes (H CZ H), diagonal U2
+    # (CZ and controlled phases as 4x4 matrices) as diagonal gates: equal to
+    # one launch per gate and to the oracle, and no U2 left as a launch of
+    # its own
+    import random as _random
+    from helpers import gate_line
+    n = 15
+    rng = _random.Random(21)
+    nrng = np.random.default_rng(21)
+    lines = []
+    for _ in range(300):
+        r = rng.random()
+        a, b = rng.sample(range(n), 2)
+        if r < 0.35:
+            lines.append(cnot_line(a, b))
+        elif r < 0.45:
+            ph = rng.uniform(0, 6.28)
+            d = [1, 1, 1, complex(np.cos(ph), np.sin(ph))]
+            m = []
+            for i in range(4):
+                for j in range(4):
+                    v = d[i] if i == j else 0
+                    m += [complex(v).real, complex(v).imag]
+            lines.append(f"U2 {min(a, b)} {max(a, b)} " + " ".join(repr(float(x)) for x in m))
+        else:
+            lines.append(gate_line(rng.choice(["U1", "H", "RX", "RZ"]), [a], rng, nrng))
+    text = "\n".join(lines) + "\n"
+    s0 = random_state(n, 22)
+    say(f"register passes: CNOT and diagonal U2 {dtype}")
+    a_ = sq.Engine(n, dtype=dtype)
+    b_ = sq.Engine(n, dtype=dtype, sweep_mode=1)
+    try:
+        a_.set_state(s0.astype(np.complex128 if dtype == "c128" else np.complex64))
+        b_.set_state(s0.astype(np.complex128 if dtype == "c128" else np.complex64))
+        gates, _, ng, _ = sq.parse_circuit(text, n)
+        a_.stats_reset()
+        a_.stats_enable(True)
+        a_.apply_gates(gates, ng)
+        st = a_.stats()
+        a_.stats_enable(False)
+        b_.apply_gates(gates, ng)
+        assert not any(k.startswith("dense2") for k in st), st
+        assert a_.max_abs_diff(b_) <= (1e-14 if dtype == "c128" else 2e-6)
+        check(a_.get_state(np.complex128), oracle.run(s0, n, text), dtype, f"cnot u2 {dtype}")
+    finally:
+        a_.close()
+        b_.close()
+
+
 def test_register_passes_every_fold_form(oracle):
     # every folded form present in the program (census) and the result equal to
     # one launch per gate and to the oracle
