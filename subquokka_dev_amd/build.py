@@ -35,7 +35,10 @@ INCLUDES = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 # branches (divergent regions are still structurized).
 SOURCE_FLAGS = {"sq_pass.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions"],
                 # MFMA results in vector registers: no accumulator-register reads per batch
-                "sq_fused.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+                "sq_fused.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+                # k_adj_multi branches on each gate's kind (uniform): plain scalar
+                # branches, not merged regions holding every kind's temporaries
+                "sq_adjoint.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions"]}
 
 HIP_SOURCES = ["sq_engine.hip", "sq_blocks.hip", "sq_fused.hip", "sq_adjoint.hip", "sq_stream.hip", "sq_pass.hip",
                "sq_sample.hip"]

@@ -195,6 +195,117 @@ __global__ void __launch_bounds__(kBlock) k_adj_gate(cx<T> *__restrict__ psi, cx
     block_sum<1>(acc, part + blockIdx.x);
 }
 
+// Up to four consecutive parameterised one-qubit gates (RX RY RZ P) on
+// distinct qubits, un-applied in one pass over psi and lambda.  Gate j's
+// generator acts on its own qubit and commutes with the other gates of the
+// group, so its term <lambda_j|P_j|psi_j> can be taken with the other gates
+// un-applied or not, in any order.  Each thread holds the 2^M amplitudes that
+// differ only in the group's qubits (local bit b <-> qubit q[b]).
+struct AdjMultiArgs {
+    uint64_t n_items;   // 2^n / 2^M
+    int kind[4];        // ADJ_RX, ADJ_RY, ADJ_RZ or ADJ_P
+    int q[4];           // the gates' qubits
+    int qs[4];          // the same, ascending (where zeros are inserted)
+    double c[4], s[4];  // cos, sin of the half angle (RX RY RZ) or of the angle (P)
+};
+
+template <typename T, int M>
+__global__ void __launch_bounds__(kBlock) k_adj_multi(cx<T> *__restrict__ psi, cx<T> *__restrict__ lam,
+                                                      const AdjMultiArgs a, double *__restrict__ part,
+                                                      uint64_t part_stride) {
+    constexpr int D = 1 << M;
+    double acc[M];
+#pragma unroll
+    for (int b = 0; b < M; b++) acc[b] = 0.0;
+    const uint64_t gstride = (uint64_t)gridDim.x * kBlock;
+    uint64_t off[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int b = 0; b < M; b++) o |= (uint64_t)((j >> b) & 1) << a.q[b];
+        off[j] = o;
+    }
+    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < a.n_items; p += gstride) {
+        uint64_t base = p;
+#pragma unroll
+        for (int b = 0; b < M; b++) base = insert_zero(base, a.qs[b]);
+        cx<T> x[D], l[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            x[j] = psi[base + off[j]];
+            l[j] = lam[base + off[j]];
+        }
+        // Gate b's term, then gate b un-applied: the later terms are unchanged,
+        // since gate b and the other gates' generators act on other qubits.
+#pragma unroll
+        for (int b = 0; b < M; b++) {
+            const T c = (T)a.c[b], s = (T)a.s[b];
+            switch (a.kind[b]) {   // uniform; one loop per kind keeps one path's temporaries live
+            case ADJ_RX:
+#pragma unroll
+                for (int j = 0; j < D; j++) {
+                    if ((j >> b) & 1) continue;
+                    const int j1 = j | (1 << b);
+                    const cx<T> x0 = x[j], x1 = x[j1], l0 = l[j], l1 = l[j1];
+                    acc[b] += ((double)l0.re * x1.im - (double)l0.im * x1.re) + ((double)l1.re * x0.im - (double)l1.im * x0.re);
+                    // RX(t)^dag = [[c, i s], [i s, c]]
+                    x[j] = {c * x0.re - s * x1.im, c * x0.im + s * x1.re};
+                    x[j1] = {c * x1.re - s * x0.im, c * x1.im + s * x0.re};
+                    l[j] = {c * l0.re - s * l1.im, c * l0.im + s * l1.re};
+                    l[j1] = {c * l1.re - s * l0.im, c * l1.im + s * l0.re};
+                }
+                break;
+            case ADJ_RY:
+#pragma unroll
+                for (int j = 0; j < D; j++) {
+                    if ((j >> b) & 1) continue;
+                    const int j1 = j | (1 << b);
+                    const cx<T> x0 = x[j], x1 = x[j1], l0 = l[j], l1 = l[j1];
+                    acc[b] += ((double)l1.re * x0.re + (double)l1.im * x0.im) - ((double)l0.re * x1.re + (double)l0.im * x1.im);
+                    // RY(t)^dag = [[c, s], [-s, c]]
+                    x[j] = {c * x0.re + s * x1.re, c * x0.im + s * x1.im};
+                    x[j1] = {c * x1.re - s * x0.re, c * x1.im - s * x0.im};
+                    l[j] = {c * l0.re + s * l1.re, c * l0.im + s * l1.im};
+                    l[j1] = {c * l1.re - s * l0.re, c * l1.im - s * l0.im};
+                }
+                break;
+            case ADJ_RZ:
+#pragma unroll
+                for (int j = 0; j < D; j++) {
+                    if ((j >> b) & 1) continue;
+                    const int j1 = j | (1 << b);
+                    const cx<T> x0 = x[j], x1 = x[j1], l0 = l[j], l1 = l[j1];
+                    acc[b] += ((double)l0.re * x0.im - (double)l0.im * x0.re) - ((double)l1.re * x1.im - (double)l1.im * x1.re);
+                    // RZ(t)^dag = diag(e^{i t/2}, e^{-i t/2})
+                    x[j] = cmul(x0, cx<T>{c, s});
+                    x[j1] = cmul(x1, cx<T>{c, -s});
+                    l[j] = cmul(l0, cx<T>{c, s});
+                    l[j1] = cmul(l1, cx<T>{c, -s});
+                }
+                break;
+            default:   // ADJ_P: dE/dt = -2 Im(conj(l1) x1); P(t)^dag = diag(1, e^{-i t})
+#pragma unroll
+                for (int j = 0; j < D; j++) {
+                    if ((j >> b) & 1) continue;
+                    const int j1 = j | (1 << b);
+                    const cx<T> x1 = x[j1], l1 = l[j1];
+                    acc[b] += -2.0 * ((double)l1.re * x1.im - (double)l1.im * x1.re);
+                    x[j1] = cmul(x1, cx<T>{c, -s});
+                    l[j1] = cmul(l1, cx<T>{c, -s});
+                }
+                break;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            psi[base + off[j]] = x[j];
+            lam[base + off[j]] = l[j];
+        }
+    }
+    block_sum<M>(acc, part + blockIdx.x, part_stride);
+}
+
 // A diagonal observable as Z strings: h(i) = sum_t coef_t (-1)^{popcount(i & mask_t)}.
 struct ZTerm {
     double coef;

@@ -369,16 +369,53 @@ int adjoint_general(sq_engine *e, const sq_gate *gates, const int32_t *param, ui
         }
         return r;
     };
+    // Consecutive one-qubit parameterised gates on distinct qubits go to one
+    // k_adj_multi launch (up to four in complex64, three in complex128);
+    // two-qubit ones (CP, RZZ) to k_adj_gate.
+    struct Pending {
+        uint64_t gi;
+        size_t pk;   // slot of its gradient partials
+    };
+    std::vector<Pending> group;   // descending pk (the sweep's order)
+    auto launch_group = [&]() -> int {
+        if (group.empty()) return SQ_OK;
+        const int M = (int)group.size();
+        AdjMultiArgs a{};
+        a.n_items = na >> M;
+        for (int b = 0; b < M; b++) {   // local bit b: the b-th smallest partial slot
+            const sq_gate &g = circ[group[M - 1 - b].gi];
+            a.kind[b] = adj_kind(g.kind);
+            a.q[b] = g.q[0];
+            const double ang = a.kind[b] == ADJ_P ? g.param : g.param / 2;
+            a.c[b] = std::cos(ang);
+            a.s[b] = std::sin(ang);
+        }
+        std::copy(a.q, a.q + M, a.qs);
+        std::sort(a.qs, a.qs + M);
+        double *dst = part + group.back().pk * grid;   // slots pk_min .. pk_min + M - 1
+        const int slot = e->stat_begin("adj_multi", 4.0 * e->amp_bytes() * (double)na);
+        switch (M) {
+        case 1: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_multi<T, 1>), grid, kBlock, 0, e->stream, psi, lam, a, dst, (uint64_t)grid); break;
+        case 2: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_multi<T, 2>), grid, kBlock, 0, e->stream, psi, lam, a, dst, (uint64_t)grid); break;
+        case 3: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_multi<T, 3>), grid, kBlock, 0, e->stream, psi, lam, a, dst, (uint64_t)grid); break;
+        default: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_multi<T, 4>), grid, kBlock, 0, e->stream, psi, lam, a, dst, (uint64_t)grid); break;
+        }
+        e->stat_end(slot);
+        group.clear();
+        return e->check_launch("k_adj_multi");
+    };
     size_t pk = pgates.size();
     for (uint64_t gi = ng; gi-- > 0;) {
         // entering the previous segment: psi is re-anchored to the FP64 copy
         // taken at this segment's end (the next one's start)
         while (seg > 1 && gi < seg_end[seg - 2]) {
+            if ((rc = launch_group())) return rc;
             if ((rc = flush())) return rc;
             if ((rc = restore(seg - 1))) return rc;
             seg--;
         }
         if (param[gi] < 0) {
+            if ((rc = launch_group())) return rc;
             sq_gate inv;
             adjoint_gate(circ[gi], inv);
             inv_run.push_back(inv);
@@ -387,13 +424,22 @@ int adjoint_general(sq_engine *e, const sq_gate *gates, const int32_t *param, ui
         if ((rc = flush())) return rc;
         pk--;
         const sq_gate &g = circ[gi];
+        const int kind = adj_kind(g.kind);
+        if (kind != ADJ_CP && kind != ADJ_RZZ) {
+            // four gates per launch for complex64 (171 VGPRs), three for complex128
+            bool clash = group.size() == (sizeof(T) == 4 ? 4u : 3u);
+            for (const auto &m : group) clash = clash || circ[m.gi].q[0] == g.q[0];
+            if (clash && (rc = launch_group())) return rc;
+            group.push_back({gi, pk});
+            continue;
+        }
+        if ((rc = launch_group())) return rc;
         AdjGateArgs a{};
-        a.kind = adj_kind(g.kind);
-        const bool two = a.kind == ADJ_CP || a.kind == ADJ_RZZ;
-        a.n_items = two ? na : na / 2;
+        a.kind = kind;
+        a.n_items = na;
         a.q0 = g.q[0];
-        a.q1 = two ? g.q[1] : 0;
-        const double ang = (a.kind == ADJ_P || a.kind == ADJ_CP) ? g.param : g.param / 2;
+        a.q1 = g.q[1];
+        const double ang = a.kind == ADJ_CP ? g.param : g.param / 2;
         a.c = std::cos(ang);
         a.s = std::sin(ang);
         // every block writes its partial (zero when it has no items)
@@ -402,6 +448,7 @@ int adjoint_general(sq_engine *e, const sq_gate *gates, const int32_t *param, ui
         e->stat_end(slot);
         if ((rc = e->check_launch("k_adj_gate"))) return rc;
     }
+    if ((rc = launch_group())) return rc;
     if ((rc = flush())) return rc;
     if (ckpt && (rc = restore(0))) return rc;   // the state the call started from, exactly
     std::vector<double> h(pgates.size() * grid);
