@@ -254,7 +254,8 @@ int sq_stats_reset(sq_engine *e);
  * reference counterpart): how many ops of each form its passes run.
  *   0 dense on a register slot, 1 dense on lane bits 0-3, 2 dense on lane bits
  *   4-5, 3 dense inside the complex64 vector, 4 diagonal without register
- *   bits, 5 diagonal with one register bit, 6 diagonal with two, 7 unused;
+ *   bits, 5 diagonal with one register bit, 6 diagonal with two, 7 products
+ *   of one-qubit phases (dense gates run as phase, real rotation, phase);
  *   8 + 4 t + c: a controlled phase folded into a dense op (complex128), target
  *   t = 0 register slot, 1 lane bits 0-3, 2 lane bits 4-5; control c = 0
  *   register slot 0, 1 another register slot, 2 a lane bit, 3 a tile bit.

@@ -76,6 +76,7 @@ struct sq_engine {
     int pass_qbits = 5;             // register qubits per pass (1..5; set by sq_create)
     int lane_moves = 3;             // top lane positions a pass may give to other qubits (0..3)
     int plan_beam = sq::kDefaultPlanBeam;   // register-pass planner: sets kept per size (1: greedy)
+    bool phase_split = true;        // register passes: one-qubit dense gates as phase, real rotation, phase
     // Register-pass schedules of recent gate lists, keyed by their structure
     // (qubits and gate classes: the schedule does not depend on angles or
     // coefficients), most recent first.

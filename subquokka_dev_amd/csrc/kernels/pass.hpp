@@ -16,9 +16,12 @@
 //     knows.
 // Each gate is its own 2x2 (or diagonal) step in FP arithmetic of the state's
 // type -- nothing is composed except a controlled phase folded into the next
-// dense gate on one of its qubits (its phase scales two coefficients) -- so a
-// pass agrees with the gates applied one launch at a time up to the rounding
-// of individual products.  Per-op work
+// dense gate on one of its qubits (its phase scales two coefficients), and a
+// dense gate with complex coefficients may arrive split into a real rotation
+// and one-qubit phases that the host collects into PO_PHASES ops (the product
+// is the gate; only the rounding differs) -- so a pass agrees with the gates
+// applied one launch at a time up to the rounding of individual products.
+// Per-op work
 // is shaped so that nothing is selected per amplitude: lane-gate
 // coefficients are chosen once per lane, shuffle masks are constants, and a
 // diagonal factor is chosen per lane, per register at compile time, or once
@@ -48,7 +51,16 @@ enum PassOpKind : int32_t {
                          //   (1: m0 real)
     PO_DENSE_LANE_C = 7, // complex<double> only: the same fold into a dense gate on lane bit a;
                          //   b, active and coef[4] as PO_DENSE_REG_C
+    PO_PHASES = 8,       // a product of one-qubit phases: amplitude i times L(i) T(i), where
+                         //   L = coef[0] * prod over virtual positions p < VB + 6 with bit p of
+                         //   a set of (bit p of i ? z_p : 1), and T = t[register index] when
+                         //   b = 1 (else 1).  c coefficient records (8 coefficients each, no
+                         //   header) follow the op: when a != 0, one with z_p in slot p; when
+                         //   b = 1, the 2^KH table t in the next ceil(2^KH / 8)
 };
+// Dense ops (PO_DENSE_REG/_C, PO_DENSE_LANE/_C, PO_DENSE_VEC) with c = 15 have
+// four real coefficients (after a fold, also the folded ones): the kernel
+// leaves out the zero imaginary terms.
 
 // 32-bit fields only: the op loop reads records with scalar (SMEM) loads,
 // which ignore the two low bits of their address components; byte fields
@@ -91,14 +103,18 @@ template <typename T> struct PassRec {
         for (int i = 0; i < NV; i++) w[i] = q[i];
     }
     __device__ int dw(int i) const { return w[i / 16][i % 16]; }
-    __device__ cx<T> coef(int j) const {
+    // coefficient j of an op record (after its eight header dwords)
+    __device__ cx<T> coef(int j) const { return at((sizeof(T) == 8 ? 2 : 4) + j); }
+    // coefficient j of a coefficient record (no header), j < 8
+    __device__ cx<T> xcoef(int j) const { return at(j); }
+    __device__ cx<T> at(int j) const {
         if constexpr (sizeof(T) == 8) {
-            const int b = 8 + 4 * j;
+            const int b = 4 * j;
             const uint64_t r = (uint64_t)(uint32_t)dw(b) | ((uint64_t)(uint32_t)dw(b + 1) << 32);
             const uint64_t m = (uint64_t)(uint32_t)dw(b + 2) | ((uint64_t)(uint32_t)dw(b + 3) << 32);
             return {__builtin_bit_cast(double, r), __builtin_bit_cast(double, m)};
         } else {
-            const int b = 8 + 2 * j;
+            const int b = 2 * j;
             return {__builtin_bit_cast(float, dw(b)), __builtin_bit_cast(float, dw(b + 1))};
         }
     }
@@ -297,6 +313,12 @@ template <typename T> __device__ inline void ip_cmul(cx<T> &x, const cx<T> f) {
     fma_into(x.re, f.re, pr);
     fma_into(x.im, f.re, pi);
 }
+// The same with a wave-uniform factor (scalar-register operands).
+template <typename T> __device__ inline void ip_cmul_s(cx<T> &x, const cx<T> f) {
+    const T pr = -(x.im * f.im), pi = x.re * f.im;
+    fma_into_s(x.re, f.re, pr);
+    fma_into_s(x.im, f.re, pi);
+}
 
 // (x, y) <- (m0 x + m1 y, m2 x + m3 y): 4 temporaries, 16 FP64 ops.  SU:
 // the coefficients are wave-uniform (see fma_into_s).
@@ -342,11 +364,20 @@ __device__ inline void ip_pair_real(cx<T> &x, cx<T> &y, const cx<T> m0, const cx
 }
 
 // x <- ms x + mp p (p: the partner's value, already in other registers).
-template <typename T> __device__ inline void ip_lane(cx<T> &x, const cx<T> p, const cx<T> ms, const cx<T> mp) {
-    const T pr = fmaT(mp.re, p.re, fmaT(-mp.im, p.im, -(ms.im * x.im)));
-    const T pi = fmaT(mp.re, p.im, fmaT(mp.im, p.re, ms.im * x.re));
-    fma_into(x.re, ms.re, pr);
-    fma_into(x.im, ms.re, pi);
+// RM = 15: both coefficients real, 4 FP64 ops instead of 8 (the terms left
+// out are exact zeros).
+template <int RM = 0, typename T>
+__device__ inline void ip_lane(cx<T> &x, const cx<T> p, const cx<T> ms, const cx<T> mp) {
+    if constexpr (RM == 15) {
+        const T pr = mp.re * p.re, pi = mp.re * p.im;
+        fma_into(x.re, ms.re, pr);
+        fma_into(x.im, ms.re, pi);
+    } else {
+        const T pr = fmaT(mp.re, p.re, fmaT(-mp.im, p.im, -(ms.im * x.im)));
+        const T pi = fmaT(mp.re, p.im, fmaT(mp.im, p.re, ms.im * x.re));
+        fma_into(x.re, ms.re, pr);
+        fma_into(x.im, ms.re, pi);
+    }
 }
 
 // One dense 2x2 on the register pairs (r, r | 2^J).  SU: coefficients the
@@ -413,7 +444,7 @@ __device__ inline void pass_dense_reg_at(Pack<T> (&v)[R], int j, const cx<T> m0,
 // chosen once per op; the partner's value comes from a DPP permutation.
 // Lane bits 4-5: each register pair (r, r | 1) swaps the lane bit with register
 // bit 0, takes the gate as a register-pair op, and swaps back.
-template <typename T, int MASK, int R, int E, bool SU = false>
+template <typename T, int MASK, int R, int E, bool SU = false, int RM = 0>
 __device__ inline void pass_dense_lane(Pack<T> (&v)[R], bool hi, const Coef4<T> k) {
     if constexpr (MASK <= 8) {
         const cx<T> ms = csel(hi, k.c3, k.c0), mp = csel(hi, k.c2, k.c1);
@@ -422,7 +453,7 @@ __device__ inline void pass_dense_lane(Pack<T> (&v)[R], bool hi, const Coef4<T> 
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const cx<T> p = {lane_xor<MASK>(v[r].e[e].re), lane_xor<MASK>(v[r].e[e].im)};
-                ip_lane(v[r].e[e], p, ms, mp);
+                ip_lane<RM>(v[r].e[e], p, ms, mp);
             }
         }
     } else {
@@ -434,7 +465,7 @@ __device__ inline void pass_dense_lane(Pack<T> (&v)[R], bool hi, const Coef4<T> 
                 cx<T> &x = v[r].e[e], &y = v[r + 1].e[e];
                 permlane_swap<MASK>(x.re, y.re);
                 permlane_swap<MASK>(x.im, y.im);
-                ip_pair<SU>(x, y, k.c0, k.c1, k.c2, k.c3);
+                ip_pair_real<RM == 15 ? 15 : 0, SU>(x, y, k.c0, k.c1, k.c2, k.c3);
                 permlane_swap<MASK>(x.re, y.re);
                 permlane_swap<MASK>(x.im, y.im);
             }
@@ -446,9 +477,10 @@ __device__ inline void pass_dense_lane(Pack<T> (&v)[R], bool hi, const Coef4<T> 
 // coefficients for control 0 / 1.  Lane bits 0-3: one uniform branch per
 // register.  Lane bits 4-5: after the swap, a pair (r, r | 1) shares its
 // control unless sj = 0, whose bit the swap has moved into lane bit MASK.
-template <typename T, int MASK, int R, int E>
+template <typename T, int MASK, int R, int E, int RM = 0>
 __device__ inline void pass_dense_lane_ctl(Pack<T> (&v)[R], bool hi, int sj, const Coef4<T> k0, const Coef4<T> k1,
                                            bool lane_bit_set) {
+    constexpr int PM = RM == 15 ? 15 : 0;
     if constexpr (MASK <= 8) {
         const cx<T> ms0 = csel(hi, k0.c3, k0.c0), mp0 = csel(hi, k0.c2, k0.c1);
         const cx<T> ms1 = csel(hi, k1.c3, k1.c0), mp1 = csel(hi, k1.c2, k1.c1);
@@ -457,8 +489,8 @@ __device__ inline void pass_dense_lane_ctl(Pack<T> (&v)[R], bool hi, int sj, con
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const cx<T> p = {lane_xor<MASK>(v[r].e[e].re), lane_xor<MASK>(v[r].e[e].im)};
-                if ((r >> sj) & 1) ip_lane(v[r].e[e], p, ms1, mp1);
-                else ip_lane(v[r].e[e], p, ms0, mp0);
+                if ((r >> sj) & 1) ip_lane<PM>(v[r].e[e], p, ms1, mp1);
+                else ip_lane<PM>(v[r].e[e], p, ms0, mp0);
             }
         }
     } else {
@@ -472,9 +504,9 @@ __device__ inline void pass_dense_lane_ctl(Pack<T> (&v)[R], bool hi, int sj, con
                 cx<T> &x = v[r].e[e], &y = v[r + 1].e[e];
                 permlane_swap<MASK>(x.re, y.re);
                 permlane_swap<MASK>(x.im, y.im);
-                if (sj == 0) ip_pair(x, y, kl.c0, kl.c1, kl.c2, kl.c3);
-                else if ((r >> sj) & 1) ip_pair(x, y, k1.c0, k1.c1, k1.c2, k1.c3);
-                else ip_pair(x, y, k0.c0, k0.c1, k0.c2, k0.c3);
+                if (sj == 0) ip_pair_real<PM>(x, y, kl.c0, kl.c1, kl.c2, kl.c3);
+                else if ((r >> sj) & 1) ip_pair_real<PM>(x, y, k1.c0, k1.c1, k1.c2, k1.c3);
+                else ip_pair_real<PM>(x, y, k0.c0, k0.c1, k0.c2, k0.c3);
                 permlane_swap<MASK>(x.re, y.re);
                 permlane_swap<MASK>(x.im, y.im);
             }
@@ -518,6 +550,32 @@ __device__ inline void pass_diag_reg2(Pack<T> (&v)[R], const Coef4<T> d, uint32_
             if ((((r >> J0) & 1) | (((r >> J1) & 1) << 1)) == l)
 #pragma unroll
                 for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], f);
+    }
+}
+
+template <typename T, int R, int E, bool SU, int RM>
+__device__ inline void pass_dense_lane_at(Pack<T> (&v)[R], int a, bool hi, const Coef4<T> k) {
+    switch (a) {
+    case 0: pass_dense_lane<T, 1, R, E, false, RM>(v, hi, k); break;
+    case 1: pass_dense_lane<T, 2, R, E, false, RM>(v, hi, k); break;
+    case 2: pass_dense_lane<T, 4, R, E, false, RM>(v, hi, k); break;
+    case 3: pass_dense_lane<T, 8, R, E, false, RM>(v, hi, k); break;
+    case 4: pass_dense_lane<T, 16, R, E, SU, RM>(v, hi, k); break;
+    case 5: pass_dense_lane<T, 32, R, E, SU, RM>(v, hi, k); break;
+    default: break;
+    }
+}
+template <typename T, int R, int E, int RM>
+__device__ inline void pass_dense_lane_ctl_at(Pack<T> (&v)[R], int a, bool hi, int sj, const Coef4<T> k0,
+                                              const Coef4<T> k1, bool lb) {
+    switch (a) {
+    case 0: pass_dense_lane_ctl<T, 1, R, E, RM>(v, hi, sj, k0, k1, lb); break;
+    case 1: pass_dense_lane_ctl<T, 2, R, E, RM>(v, hi, sj, k0, k1, lb); break;
+    case 2: pass_dense_lane_ctl<T, 4, R, E, RM>(v, hi, sj, k0, k1, lb); break;
+    case 3: pass_dense_lane_ctl<T, 8, R, E, RM>(v, hi, sj, k0, k1, lb); break;
+    case 4: pass_dense_lane_ctl<T, 16, R, E, RM>(v, hi, sj, k0, k1, lb); break;
+    case 5: pass_dense_lane_ctl<T, 32, R, E, RM>(v, hi, sj, k0, k1, lb); break;
+    default: break;
     }
 }
 
@@ -587,27 +645,22 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                     const cx<T> m1p = {m1.re * ph.re - m1.im * ph.im, m1.re * ph.im + m1.im * ph.re};
                     const cx<T> m3p = {m3.re * ph.re - m3.im * ph.im, m3.re * ph.im + m3.im * ph.re};
                     if (o.active == 0) {   // control in a register slot
-                        if (o.c == 1) pass_dense_reg_ctl_at<T, KH, R, E, 1>(v, o.a, o.b, m0, m1, m2, m3, m1p, m3p);
+                        if (o.c == 15) pass_dense_reg_ctl_at<T, KH, R, E, 15>(v, o.a, o.b, m0, m1, m2, m3, m1p, m3p);
+                        else if (o.c == 1) pass_dense_reg_ctl_at<T, KH, R, E, 1>(v, o.a, o.b, m0, m1, m2, m3, m1p, m3p);
                         else pass_dense_reg_ctl_at<T, KH, R, E, 0>(v, o.a, o.b, m0, m1, m2, m3, m1p, m3p);
                     } else {               // lane bit (per lane) or tile bit (per wave)
                         const bool on = o.active == 1 ? ((base >> o.b) & 1) != 0 : ((tb >> o.b) & 1) != 0;
                         const cx<T> m1s = csel(on, m1p, m1), m3s = csel(on, m3p, m3);
-                        if (o.c == 1) pass_dense_reg_at<T, KH, R, E, 1>(v, o.a, m0, m1s, m2, m3s);
+                        if (o.c == 15) pass_dense_reg_at<T, KH, R, E, 15>(v, o.a, m0, m1s, m2, m3s);
+                        else if (o.c == 1) pass_dense_reg_at<T, KH, R, E, 1>(v, o.a, m0, m1s, m2, m3s);
                         else pass_dense_reg_at<T, KH, R, E, 0>(v, o.a, m0, m1s, m2, m3s);
                     }
                 }
                 break; }
             case PO_DENSE_LANE: {
                 const bool hi = (lane >> o.a) & 1;
-                switch (o.a) {
-                case 0: pass_dense_lane<T, 1, R, E>(v, hi, c); break;
-                case 1: pass_dense_lane<T, 2, R, E>(v, hi, c); break;
-                case 2: pass_dense_lane<T, 4, R, E>(v, hi, c); break;
-                case 3: pass_dense_lane<T, 8, R, E>(v, hi, c); break;
-                case 4: pass_dense_lane<T, 16, R, E, true>(v, hi, c); break;
-                case 5: pass_dense_lane<T, 32, R, E, true>(v, hi, c); break;
-                default: break;
-                }
+                if (o.c == 15) pass_dense_lane_at<T, R, E, true, 15>(v, o.a, hi, c);
+                else pass_dense_lane_at<T, R, E, true, 0>(v, o.a, hi, c);
                 break; }
             case PO_DENSE_LANE_C: {
                 if constexpr (sizeof(T) == 8) {
@@ -618,36 +671,70 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                     const Coef4<T> k1 = {c.c0, m1p, c.c2, m3p};
                     if (o.active == 0) {   // control in register slot o.b
                         const bool lb = ((lane >> o.a) & 1) != 0;
-                        switch (o.a) {
-                        case 0: pass_dense_lane_ctl<T, 1, R, E>(v, hi, o.b, c, k1, lb); break;
-                        case 1: pass_dense_lane_ctl<T, 2, R, E>(v, hi, o.b, c, k1, lb); break;
-                        case 2: pass_dense_lane_ctl<T, 4, R, E>(v, hi, o.b, c, k1, lb); break;
-                        case 3: pass_dense_lane_ctl<T, 8, R, E>(v, hi, o.b, c, k1, lb); break;
-                        case 4: pass_dense_lane_ctl<T, 16, R, E>(v, hi, o.b, c, k1, lb); break;
-                        case 5: pass_dense_lane_ctl<T, 32, R, E>(v, hi, o.b, c, k1, lb); break;
-                        default: break;
-                        }
+                        if (o.c == 15) pass_dense_lane_ctl_at<T, R, E, 15>(v, o.a, hi, o.b, c, k1, lb);
+                        else pass_dense_lane_ctl_at<T, R, E, 0>(v, o.a, hi, o.b, c, k1, lb);
                     } else {               // lane bit (per lane) or tile bit (per wave)
                         const bool on = o.active == 1 ? ((base >> o.b) & 1) != 0 : ((tb >> o.b) & 1) != 0;
                         const Coef4<T> k = {c.c0, csel(on, m1p, c.c1), c.c2, csel(on, m3p, c.c3)};
-                        switch (o.a) {
-                        case 0: pass_dense_lane<T, 1, R, E>(v, hi, k); break;
-                        case 1: pass_dense_lane<T, 2, R, E>(v, hi, k); break;
-                        case 2: pass_dense_lane<T, 4, R, E>(v, hi, k); break;
-                        case 3: pass_dense_lane<T, 8, R, E>(v, hi, k); break;
-                        case 4: pass_dense_lane<T, 16, R, E>(v, hi, k); break;
-                        case 5: pass_dense_lane<T, 32, R, E>(v, hi, k); break;
-                        default: break;
-                        }
+                        if (o.c == 15) pass_dense_lane_at<T, R, E, false, 15>(v, o.a, hi, k);
+                        else pass_dense_lane_at<T, R, E, false, 0>(v, o.a, hi, k);
                     }
                 }
                 break; }
             case PO_DENSE_VEC: {
                 if constexpr (E == 2) {
                     const cx<T> m0 = c.c0, m1 = c.c1, m2 = c.c2, m3 = c.c3;
+                    if (o.c == 15) {
 #pragma unroll
-                    for (int r = 0; r < R; r++) ip_pair<true>(v[r].e[0], v[r].e[1], m0, m1, m2, m3);
+                        for (int r = 0; r < R; r++) ip_pair_real<15, true>(v[r].e[0], v[r].e[1], m0, m1, m2, m3);
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < R; r++) ip_pair<true>(v[r].e[0], v[r].e[1], m0, m1, m2, m3);
+                    }
                 }
+                break; }
+            case PO_PHASES: {
+                // L per lane (and vector element): the scalar times z_p for each
+                // marked position whose bit is set
+                cx<T> L[E];
+#pragma unroll
+                for (int e = 0; e < E; e++) L[e] = c.c0;
+                int x = op + 1;   // the next coefficient record
+                if (o.a) {
+                    const PassRec<T> zr(a.ops + x);
+                    x++;
+#pragma unroll
+                    for (int p = 0; p < S; p++) {
+                        if (!((o.a >> p) & 1)) continue;
+                        const cx<T> z = zr.xcoef(p);
+#pragma unroll
+                        for (int e = 0; e < E; e++) L[e] = csel(((base + e) >> p) & 1, cmul(L[e], z), L[e]);
+                    }
+                }
+                if (o.b) {         // register table: 8 entries a record
+#pragma unroll
+                    for (int k = 0; k < (R + 7) / 8; k++) {
+                        const PassRec<T> tr(a.ops + x + k);
+#pragma unroll
+                        for (int j = 0; j < 8 && k * 8 + j < R; j++) {
+                            const int r = k * 8 + j;
+                            const cx<T> t = tr.xcoef(j);
+                            if (o.a) {
+#pragma unroll
+                                for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], cmul(L[e], t));
+                            } else {   // the scalar is in the table: a wave-uniform factor per register
+#pragma unroll
+                                for (int e = 0; e < E; e++) ip_cmul_s(v[r].e[e], t);
+                            }
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; r++)
+#pragma unroll
+                        for (int e = 0; e < E; e++) ip_cmul(v[r].e[e], L[e]);
+                }
+                op += o.c;   // past the coefficient records
                 break; }
             case PO_DIAG_FREE: {
                 // the factor depends only on lane, tile and vector bits: one per lane (and element)

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -51,17 +52,19 @@ void encode_pass_op(const LoweredGate &g, const std::vector<int> &regs, std::vec
     if (g.cls == OpClass::Dense) {
         for (int i = 0; i < 4; i++) push(g.m[i]);
         const int t = g.t[0];
+        // imaginary parts that are exactly zero as stored (in T)
+        bool all_real = true;
+        for (int i = 0; i < 4; i++) all_real = all_real && o.coef[i].im == (T)0;
         if (t < VB) {
             o.kind = PO_DENSE_VEC;
+            o.c = all_real ? 15 : 0;
         } else if (t < S) {
             o.kind = PO_DENSE_LANE;
             o.a = t - VB;
+            o.c = all_real ? 15 : 0;
         } else {
             o.kind = PO_DENSE_REG;
             o.a = slot(t);
-            // imaginary parts that are exactly zero as stored (in T)
-            bool all_real = true;
-            for (int i = 0; i < 4; i++) all_real = all_real && o.coef[i].im == (T)0;
             o.c = all_real ? 15 : (o.coef[0].im == (T)0 ? 1 : 0);
         }
         ops.push_back(o);
@@ -135,12 +138,117 @@ template <typename T> double executed_flops(const PassOp<T> &o) {
     case PO_DENSE_REG_C: return o.c == 15 ? 6.0 : (o.c == 1 ? 12.0 : 14.0);
     case PO_DENSE_LANE:
     case PO_DENSE_LANE_C:
-    case PO_DENSE_VEC: return 14.0;
+    case PO_DENSE_VEC: return o.c == 15 ? 6.0 : 14.0;
     case PO_DIAG_FREE: return o.c ? 6.0 * quarter : 6.0;
     case PO_DIAG_REG1: return o.c ? 6.0 * quarter : 3.0 * (((o.active & 3u) != 0) + ((o.active & 12u) != 0));
     case PO_DIAG_REG2: return 6.0 * quarter;
+    // the lane factor (or the scalar alone) and the register table: 6 each;
+    // forming L x t per amplitude when both are present: 6 more
+    case PO_PHASES: return o.b ? (o.a ? 12.0 : 6.0) : 6.0;
     default: return 0.0;
     }
+}
+
+// A one-qubit dense gate as phase, real rotation, phase:
+//   U = g diag(1, zl) R diag(1, zr),   R = [[c, -s], [s, c]],  c, s >= 0,
+// i.e. U = g [[c, -s zr], [s zl, c zl zr]].  Applied as R alone, the gate
+// costs 8 FP64 operations per pair instead of 14-16; the phases of all qubits
+// of a pass that fall due together go into one PO_PHASES op.  ok = false when
+// the coefficients are already real, or when the product of the factors
+// differs from U by more than 1e-15 (a matrix that is not unitary).
+struct PhaseSplit {
+    bool ok = false;
+    std::complex<double> g{1.0, 0.0}, zl{1.0, 0.0}, zr{1.0, 0.0};
+    double c = 1.0, s = 0.0;
+};
+PhaseSplit phase_split(const LoweredGate &u) {
+    PhaseSplit p;
+    if (u.cls != OpClass::Dense || u.k != 1) return p;
+    bool real = true;
+    for (int i = 0; i < 4; i++) real = real && u.m[i].imag() == 0.0;
+    if (real) return p;
+    const std::complex<double> u00 = u.m[0], u01 = u.m[1], u10 = u.m[2], u11 = u.m[3];
+    const double c = std::abs(u00), s = std::abs(u10);
+    std::complex<double> g, zl(1.0, 0.0), zr(1.0, 0.0);
+    if (c > 0) g = u00 / c;
+    else if (s > 0) g = u10 / s;   // zl = 1
+    else return p;
+    if (s > 0) {
+        if (c > 0) zl = u10 * std::conj(g) / s;
+        zr = -u01 * std::conj(g) / s;
+    } else {
+        zr = u11 * std::conj(g) / c;
+    }
+    const std::complex<double> w[4] = {g * c, -g * s * zr, g * s * zl, g * c * zl * zr};
+    for (int i = 0; i < 4; i++)
+        if (std::abs(w[i] - u.m[i]) > 1e-15) return p;
+    p.ok = true;
+    p.g = g; p.zl = zl; p.zr = zr; p.c = c; p.s = s;
+    return p;
+}
+
+// Appends a PO_PHASES op (and its coefficient records): fac[p] is the factor
+// diag(1, fac[p]) of virtual position p, for p < S and the register slots;
+// gs a scalar.  A single register factor goes into a PO_DIAG_REG1 op of its
+// own (half the registers), two or more into the 2^KH table.  Returns the FP
+// operations per amplitude.
+template <typename T>
+double emit_phases(const std::vector<std::complex<double>> &fac, const std::vector<int> &regs, int S,
+                   std::complex<double> gs, std::vector<PassOp<T>> &ops) {
+    static_assert(sizeof(PassOp<T>) == 8 * sizeof(cx<T>), "a coefficient record holds 8 coefficients");
+    const std::complex<double> one(1.0, 0.0);
+    int lane_mask = 0;
+    for (int p = 0; p < S; p++)
+        if (fac[p] != one) lane_mask |= 1 << p;
+    std::vector<int> js;
+    for (int j = 0; j < (int)regs.size(); j++)
+        if (fac[regs[j]] != one) js.push_back(j);
+    double flops = 0;
+    if (js.size() == 1) {   // e[(register bit << 1)]: 1 for bit 0, the factor for bit 1
+        PassOp<T> o{};
+        o.kind = PO_DIAG_REG1;
+        o.a = js[0];
+        o.b = -1;
+        o.c = 1;
+        const std::complex<double> z = fac[regs[js[0]]];
+        o.coef[0] = o.coef[1] = {(T)1, (T)0};
+        o.coef[2] = o.coef[3] = {(T)z.real(), (T)z.imag()};
+        o.active = 12u;
+        ops.push_back(o);
+        flops += executed_flops(o);
+    }
+    const bool table = js.size() >= 2;
+    if (!lane_mask && !table && gs == one) return flops;
+    auto rec = [](const std::complex<double> *z, int n) {
+        PassOp<T> r{};
+        cx<T> w[8];
+        for (int i = 0; i < 8; i++) w[i] = {(T)(i < n ? z[i].real() : 1.0), (T)(i < n ? z[i].imag() : 0.0)};
+        std::memcpy(static_cast<void *>(&r), w, sizeof w);
+        return r;
+    };
+    PassOp<T> o{};
+    o.kind = PO_PHASES;
+    o.a = lane_mask;
+    o.b = table;
+    const int R = 1 << regs.size();
+    o.c = (lane_mask ? 1 : 0) + (table ? (R + 7) / 8 : 0);
+    const std::complex<double> l0 = lane_mask || !table ? gs : one;   // else the scalar goes into the table
+    o.coef[0] = {(T)l0.real(), (T)l0.imag()};
+    ops.push_back(o);
+    flops += executed_flops(o);
+    if (lane_mask) {
+        std::complex<double> z[8];
+        for (int p = 0; p < 8; p++) z[p] = p < S ? fac[p] : one;
+        ops.push_back(rec(z, 8));
+    }
+    if (table) {
+        std::vector<std::complex<double>> t(R, lane_mask ? one : gs);
+        for (int r = 0; r < R; r++)
+            for (int j : js)
+                if ((r >> j) & 1) t[r] *= fac[regs[j]];
+        for (int k = 0; k < (R + 7) / 8; k++) ops.push_back(rec(t.data() + 8 * k, std::min(8, R - 8 * k)));
+    }
+    return flops;
 }
 
 template <typename T, int KH>
@@ -264,23 +372,108 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
                 }
             }
         }
+        // Phase split (phase_split above): each dense gate's rotation runs
+        // as R; its phases wait per qubit in pend[] until a flush applies all
+        // factors due -- placed just before the first gate that needs one, it
+        // also takes, for every other resident qubit, the right factor of that
+        // qubit's next gate and the one-qubit diagonals before it (all of them
+        // diagonal, so they commute up to that gate).  The scalars collect in
+        // gpass; the pass ends with a flush of what is left.
+        const std::complex<double> one(1.0, 0.0);
+        std::vector<PhaseSplit> sp(lg.size());
+        bool any_split = false;
+        if (e->phase_split)
+            for (size_t i = 0; i < lg.size(); i++) {
+                sp[i] = phase_split(lg[i]);
+                any_split = any_split || sp[i].ok;
+            }
+        auto resident = [&](int t) { return t < S || std::find(regs.begin(), regs.end(), t) != regs.end(); };
+        std::vector<std::complex<double>> pend(n, one);
+        std::vector<char> absorbed(lg.size(), 0), zr_done(lg.size(), 0);
+        std::complex<double> gpass = one;
+        auto diag1 = [&](size_t j) {
+            const LoweredGate &d = lg[j];
+            return any_split && d.cls == OpClass::Diag && d.k == 1 && resident(d.t[0]) && d.m[0] != 0.0;
+        };
+        auto flush = [&](size_t from, bool last) {
+            std::vector<std::complex<double>> fac(n, one);
+            auto take = [&](int q) {
+                std::complex<double> f = pend[q];
+                pend[q] = one;
+                for (size_t j = from; j < lg.size(); j++) {
+                    const LoweredGate &u = lg[j];
+                    if (u.cls == OpClass::Diag) {
+                        if (diag1(j) && u.t[0] == q && !absorbed[j]) {
+                            f *= u.m[1] / u.m[0];
+                            gpass *= u.m[0];
+                            absorbed[j] = 1;
+                        }
+                        continue;
+                    }
+                    if (u.t[0] != q) continue;
+                    if (sp[j].ok && !zr_done[j]) {
+                        f *= sp[j].zr;
+                        zr_done[j] = 1;
+                    }
+                    break;
+                }
+                fac[q] = f;
+            };
+            for (int p = 0; p < S; p++) take(p);
+            for (int r : regs) take(r);
+            l.flops_per_amp += emit_phases<T>(fac, regs, S, last ? gpass : one, ops);
+        };
         for (size_t i = 0; i < lg.size(); i++) {
             if (fold_into[i] >= 0) continue;   // applied inside the dense gate's op: no work of its own
-            encode_pass_op<T>(lg[i], regs, ops);
+            if (diag1(i)) {                    // a factor of its qubit's pending phase
+                if (!absorbed[i]) {
+                    pend[lg[i].t[0]] *= lg[i].m[1] / lg[i].m[0];
+                    gpass *= lg[i].m[0];
+                    absorbed[i] = 1;
+                }
+                continue;
+            }
+            LoweredGate gi = lg[i];
+            if (any_split && gi.cls == OpClass::Dense) {
+                const int q = gi.t[0];
+                std::complex<double> need = pend[q];
+                if (sp[i].ok && !zr_done[i]) need *= sp[i].zr;
+                if (need != one) flush(i, false);
+                pend[q] = one;
+                zr_done[i] = 1;
+                if (sp[i].ok) {
+                    gi.m[0] = sp[i].c;
+                    gi.m[1] = -sp[i].s;
+                    gi.m[2] = sp[i].s;
+                    gi.m[3] = sp[i].c;
+                    pend[q] = sp[i].zl;
+                    gpass *= sp[i].g;
+                    if (sp[i].c == 1.0 && sp[i].s == 0.0 && folded_from[i] < 0) continue;   // R = I
+                }
+            }
+            encode_pass_op<T>(gi, regs, ops);
             if constexpr (sizeof(T) == 8) if (folded_from[i] >= 0) {
                 const LoweredGate &d = lg[folded_from[i]];
                 PassOp<T> &o = ops.back();
-                const int ctl = lg[i].t[0] == d.t[0] ? d.t[1] : d.t[0];
+                const int ctl = gi.t[0] == d.t[0] ? d.t[1] : d.t[0];
                 const auto it = std::find(regs.begin(), regs.end(), ctl);
                 const bool lane_target = o.kind == PO_DENSE_LANE;
                 o.kind = lane_target ? PO_DENSE_LANE_C : PO_DENSE_REG_C;
                 if (it != regs.end()) { o.active = 0; o.b = (int)(it - regs.begin()); }
                 else { o.active = ctl < S ? 1 : 2; o.b = ctl; }
-                o.c = !lane_target && o.coef[0].im == (T)0 ? 1 : 0;
-                o.coef[kPassFoldPhase] = {(T)d.m[3].real(), (T)d.m[3].imag()};
+                // A real gate stays real with a real phase.  A phase within one
+                // ulp of the real axis counts as real: CP(pi) evaluates to
+                // e^{i pi} = -1 + 1.2e-16 i, and leaving out that term changes
+                // each amplitude by less than one ulp of its magnitude.
+                const std::complex<double> ph = d.m[3];
+                const bool ph_real = std::abs(ph.imag()) <= 0x1p-52 * std::abs(ph.real());
+                const bool real = o.c == 15 && ph_real;
+                o.c = real ? 15 : (!lane_target && o.coef[0].im == (T)0 ? 1 : 0);
+                o.coef[kPassFoldPhase] = {(T)ph.real(), real ? (T)0 : (T)ph.imag()};
             }
             l.flops_per_amp += executed_flops(ops.back());
         }
+        if (any_split) flush(lg.size(), true);
         l.n_ops = (int)ops.size() - l.op_begin;
         for (int j = 0; j < kh; j++) l.hpos[j] = regs[j];
         prog.launches.push_back(l);
@@ -288,7 +481,8 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
 
     // op census of this program (sq_pass_census)
     e->pass_census.fill(0);
-    for (const auto &o : ops) {
+    for (size_t oi = 0; oi < ops.size(); oi++) {
+        const PassOp<T> &o = ops[oi];
         int idx = -1;
         switch (o.kind) {
         case PO_DENSE_REG: idx = 0; break;
@@ -297,6 +491,7 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
         case PO_DIAG_FREE: idx = 4; break;
         case PO_DIAG_REG1: idx = 5; break;
         case PO_DIAG_REG2: idx = 6; break;
+        case PO_PHASES: idx = 7; oi += o.c; break;   // past its coefficient records
         case PO_DENSE_REG_C:
         case PO_DENSE_LANE_C: {
             const int t = o.kind == PO_DENSE_REG_C ? 0 : (o.a < 4 ? 1 : 2);

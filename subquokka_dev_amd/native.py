@@ -320,14 +320,17 @@ class Engine:
     def pass_census(self):
         """Op counts of the last register-pass program by form (sq_pass_census):
         a dict with 'dense_reg', 'dense_lane03', 'dense_lane45', 'dense_vec',
-        'diag_free', 'diag_reg1', 'diag_reg2' and 'fold' -- a 3x4 array of folded
-        controlled phases by target (register, lane 0-3, lane 4-5) and control
-        (register slot 0, other slot, lane, tile)."""
+        'diag_free', 'diag_reg1', 'diag_reg2', 'phases' (products of one-qubit
+        phases left by dense gates split into phase, real rotation, phase) and
+        'fold' -- a 3x4 array of folded controlled phases by target (register,
+        lane 0-3, lane 4-5) and control (register slot 0, other slot, lane,
+        tile)."""
         c = (ctypes.c_int64 * 20)()
         n = ctypes.c_int32()
         self._c(lib().sq_pass_census(self.h, c, 20, ctypes.byref(n)))
         v = list(c)
-        names = ["dense_reg", "dense_lane03", "dense_lane45", "dense_vec", "diag_free", "diag_reg1", "diag_reg2"]
+        names = ["dense_reg", "dense_lane03", "dense_lane45", "dense_vec", "diag_free", "diag_reg1", "diag_reg2",
+                 "phases"]
         out = {k: v[i] for i, k in enumerate(names)}
         out["fold"] = np.array(v[8:20], dtype=np.int64).reshape(3, 4)
         return out

@@ -74,6 +74,7 @@ int main(int argc, char **argv) {
     e->lane_moves = 0;
     if (const char *lm = std::getenv("EMUL_LANE_MOVES")) e->lane_moves = std::atoi(lm);
     if (const char *bp = std::getenv("EMUL_BITPERM")) e->bitperm_ok = std::atoi(bp) != 0;
+    if (const char *ps = std::getenv("EMUL_PHASE_SPLIT")) e->phase_split = std::atoi(ps) != 0;
     if (mode == "adjoint") {   // chunk argument = layers; theta from a fixed formula
         const int layers = chunk;
         std::vector<double> theta((size_t)2 * n * layers), grad(theta.size());

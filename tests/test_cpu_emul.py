@@ -309,6 +309,48 @@ def test_register_pass_numerics(emul, tmp_path, oracle, dtype, n, first_dense, k
     assert np.max(np.abs(got - want)) <= tol
 
 
+@pytest.mark.parametrize("dtype,first_dense", [("c128", 6), ("c64", 7)])
+def test_register_pass_phase_split(emul, tmp_path, oracle, dtype, first_dense):
+    # Dense gates with complex coefficients run as phase, real rotation, phase;
+    # the phases of all resident qubits are applied together (PO_PHASES, census
+    # slot 7).  Split and unsplit programs against the oracle and each other,
+    # with the edge cases of the split written out as matrices: a phase-only
+    # gate (the rotation is the identity and no op is left), an anti-diagonal
+    # one (cos 0), a matrix that is not unitary (kept whole), and one-qubit
+    # diagonals on lane and tile qubits between them.
+    import math
+    n = 13
+    text = register_pass_circuit(n, 200, 77, first_dense)
+    a, b = 0.7, -1.9
+    extra = []
+    for q in range(first_dense, n):
+        extra.append(f"U1 {q} 1 0 0 0 0 0 {math.cos(a)!r} {math.sin(a)!r}")
+        extra.append(f"U1 {q} 0 0 {math.cos(b)!r} {math.sin(b)!r} {math.cos(a)!r} {math.sin(a)!r} 0 0")
+        extra.append(f"P {q % first_dense} {0.3 + q!r}")
+        extra.append(f"U1 {q} 0.5 0.1 0.2 0.3 0.4 0.5 0.6 0.7")
+        extra.append(f"RZ {n - 1 - (q % 3)} {0.2 * q!r}")
+    text = text + "\n".join(extra) + "\n" + text
+    want = oracle.run(sq.gen_state(n, n), n, text)
+    tol = 1e-12 if dtype == "c128" else 2e-5   # the unitary-breaking matrix scales the state up
+    got = {}
+    for split in (1, 0):
+        f = tmp_path / "c.txt"
+        f.write_text(text)
+        out = tmp_path / f"s{split}.bin"
+        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", EMUL_CENSUS="1",
+                   EMUL_INIT_SEED=str(n), EMUL_PHASE_SPLIT=str(split))
+        r = subprocess.run([emul, "plain", str(n), "4", dtype, str(f), str(out)], capture_output=True, text=True,
+                           timeout=900, env=env)
+        assert r.returncode == 0, (r.stdout + r.stderr)[-2000:]
+        census = [int(x) for x in next(l for l in r.stdout.splitlines() if l.startswith("census")).split()[1:]]
+        assert (census[7] > 0) == bool(split), census
+        raw = np.fromfile(out, dtype=np.float64)
+        got[split] = raw[0::2] + 1j * raw[1::2]
+        scale = np.max(np.abs(want))
+        assert np.max(np.abs(got[split] - want)) <= tol * max(1.0, scale), (split, np.max(np.abs(got[split] - want)))
+    assert np.max(np.abs(got[1] - got[0])) <= (1e-13 if dtype == "c128" else 2e-5) * max(1.0, np.max(np.abs(want)))
+
+
 def test_out_of_core_direct_io_matches_pinned(emul, tmp_path, odirect_state_dir):
     # runner_type DirectIO: the state files opened O_DIRECT (create_file,
     # simulator.cpp:71-86), every transfer a 4 KiB-aligned run of the staging
