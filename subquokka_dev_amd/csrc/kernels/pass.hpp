@@ -57,6 +57,17 @@ enum PassOpKind : int32_t {
                          //   b = 1 (else 1).  c coefficient records (8 coefficients each, no
                          //   header) follow the op: when a != 0, one with z_p in slot p; when
                          //   b = 1, the 2^KH table t in the next ceil(2^KH / 8)
+    PO_ROT_REG = 9,      // the rotation [[c, -s], [s, c]] (c, s >= 0) on register slot a (or,
+                         //   with c = 1, inside the complex64 vector) as three shears:
+                         //   x += k y, y += m x, x += k y, k = coef[0].re = -s / (1 + c),
+                         //   m = coef[0].im = s; each an FMA into its own input's registers
+    PO_ROT_LANE = 10,    // the same rotation on lane bit a.  Lane bits 0-3, scaled by the host
+                         //   (the scale goes into a later PO_PHASES scalar): c = 0:
+                         //   [[1, -k], [k, 1]] with k = s / c, c = 1: [[k, -1], [1, k]] with
+                         //   k = c / s, k = coef[0].re.  Lane bits 4-5: shears as PO_ROT_REG
+    PO_SIGN = 11,        // amplitude i negated where every bit of i named below is 1 (CP(pi),
+                         //   CZ): a = mask of virtual positions < VB + 6, b = mask of register
+                         //   slots, pad[0], pad[1] = tile positions (-1: none)
 };
 // Dense ops (PO_DENSE_REG/_C, PO_DENSE_LANE/_C, PO_DENSE_VEC) with c = 15 have
 // four real coefficients (after a fold, also the folded ones): the kernel
@@ -272,6 +283,25 @@ template <typename T> __device__ inline void fma_into(T &x, const T m, const T p
 #endif
 }
 
+// x += k y into x's own registers (k wave-uniform: scalar operand; fma_acc_v:
+// k per lane).
+template <typename T> __device__ inline void fma_acc_s(T &x, const T k, const T y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(T) == 8) asm("v_fma_f64 %0, %1, %2, %0" : "+v"(x) : "s"(k), "v"(y));
+    else asm("v_fma_f32 %0, %1, %2, %0" : "+v"(x) : "s"(k), "v"(y));
+#else
+    x = fmaT(k, y, x);
+#endif
+}
+template <typename T> __device__ inline void fma_acc_v(T &x, const T k, const T y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(T) == 8) asm("v_fma_f64 %0, %1, %2, %0" : "+v"(x) : "v"(k), "v"(y));
+    else asm("v_fma_f32 %0, %1, %2, %0" : "+v"(x) : "v"(k), "v"(y));
+#else
+    x = fmaT(k, y, x);
+#endif
+}
+
 // The same with a wave-uniform multiplier read from scalar registers: the
 // FMA then reads two vector operands instead of three (and needs no copy of
 // the coefficient into vector registers).  Only for values that are the same
@@ -380,6 +410,98 @@ __device__ inline void ip_lane(cx<T> &x, const cx<T> p, const cx<T> ms, const cx
     }
 }
 
+// A rotation [[c, -s], [s, c]] (c, s >= 0) on a pair as three shears,
+// [[1, k], [0, 1]] [[1, 0], [m, 1]] [[1, k], [0, 1]] with k = -s / (1 + c) and
+// m = s (their product is the rotation when c^2 + s^2 = 1; |k| <= 1): six
+// FMAs per pair, each written over its own input, against eight operations
+// for the rotation's 2x2 product.
+template <typename T> __device__ inline void ip_shear3(cx<T> &x, cx<T> &y, const T k, const T m) {
+    fma_acc_s(x.re, k, y.re);
+    fma_acc_s(x.im, k, y.im);
+    fma_acc_s(y.re, m, x.re);
+    fma_acc_s(y.im, m, x.im);
+    fma_acc_s(x.re, k, y.re);
+    fma_acc_s(x.im, k, y.im);
+}
+template <typename T, int J, int R, int E>
+__device__ inline void pass_rot_reg(Pack<T> (&v)[R], const T k, const T m) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if ((r >> J) & 1) continue;
+#pragma unroll
+        for (int e = 0; e < E; e++) ip_shear3(v[r].e[e], v[r | (1 << J)].e[e], k, m);
+    }
+}
+template <typename T, int KH, int R, int E>
+__device__ inline void pass_rot_reg_at(Pack<T> (&v)[R], int j, const T k, const T m) {
+    switch (j) {
+    case 0: pass_rot_reg<T, 0, R, E>(v, k, m); break;
+    case 1: if constexpr (KH > 1) pass_rot_reg<T, 1, R, E>(v, k, m); break;
+    case 2: if constexpr (KH > 2) pass_rot_reg<T, 2, R, E>(v, k, m); break;
+    case 3: if constexpr (KH > 3) pass_rot_reg<T, 3, R, E>(v, k, m); break;
+    case 4: if constexpr (KH > 4) pass_rot_reg<T, 4, R, E>(v, k, m); break;
+    default: break;
+    }
+}
+
+// Sign flip of a value where the lane's mask word is 0x80000000.
+template <typename T> __device__ inline void flip_sign(cx<T> &x, uint32_t mk) {
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t m = (uint64_t)mk << 32;
+        x.re = __builtin_bit_cast(T, __builtin_bit_cast(uint64_t, x.re) ^ m);
+        x.im = __builtin_bit_cast(T, __builtin_bit_cast(uint64_t, x.im) ^ m);
+    } else {
+        x.re = __builtin_bit_cast(T, __builtin_bit_cast(uint32_t, x.re) ^ mk);
+        x.im = __builtin_bit_cast(T, __builtin_bit_cast(uint32_t, x.im) ^ mk);
+    }
+}
+
+// PO_ROT_LANE on lane bits 0-3, scaled (see PO_ROT_LANE).  form 0: the lane
+// with the bit clear gets x - k p, its partner x + k p: one FMA per part.
+// form 1: k x - p and k x + p: the partner's value is negated on the lanes
+// with the bit clear (sign bits), then one FMA per part.
+template <typename T, int MASK, int R, int E>
+__device__ inline void pass_rot_lane(Pack<T> (&v)[R], bool hi, int form, const T k) {
+    if (form == 0) {
+        const T kl = hi ? k : -k;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const cx<T> p = {lane_xor<MASK>(v[r].e[e].re), lane_xor<MASK>(v[r].e[e].im)};
+                fma_acc_v(v[r].e[e].re, kl, p.re);
+                fma_acc_v(v[r].e[e].im, kl, p.im);
+            }
+    } else {
+        const uint32_t mk = hi ? 0u : 0x80000000u;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                cx<T> p = {lane_xor<MASK>(v[r].e[e].re), lane_xor<MASK>(v[r].e[e].im)};
+                flip_sign(p, mk);
+                fma_into_s(v[r].e[e].re, k, p.re);
+                fma_into_s(v[r].e[e].im, k, p.im);
+            }
+    }
+}
+// PO_ROT_LANE on lane bits 4-5: swap with register bit 0, shears, swap back.
+template <typename T, int MASK, int R, int E>
+__device__ inline void pass_rot_lane_swap(Pack<T> (&v)[R], const T k, const T m) {
+#pragma unroll
+    for (int r = 0; r < R; r += 2) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            cx<T> &x = v[r].e[e], &y = v[r + 1].e[e];
+            permlane_swap<MASK>(x.re, y.re);
+            permlane_swap<MASK>(x.im, y.im);
+            ip_shear3(x, y, k, m);
+            permlane_swap<MASK>(x.re, y.re);
+            permlane_swap<MASK>(x.im, y.im);
+        }
+    }
+}
+
 // One dense 2x2 on the register pairs (r, r | 2^J).  SU: coefficients the
 // same in every lane.
 template <typename T, int J, int R, int E, int RM = 0, bool SU = false>
@@ -402,13 +524,10 @@ __device__ inline void pass_dense_reg_ctl(Pack<T> (&v)[R], int sj, const cx<T> m
     for (int r = 0; r < R; r++) {
         if ((r >> J) & 1) continue;
         const int q = r | (1 << J);
-        if ((r >> sj) & 1) {
+        const bool c = (r >> sj) & 1;   // wave-uniform: scalar selects, one copy of the code
+        const cx<T> a1 = {c ? m1p.re : m1.re, c ? m1p.im : m1.im}, a3 = {c ? m3p.re : m3.re, c ? m3p.im : m3.im};
 #pragma unroll
-            for (int e = 0; e < E; e++) ip_pair_real<RM, true>(v[r].e[e], v[q].e[e], m0, m1p, m2, m3p);
-        } else {
-#pragma unroll
-            for (int e = 0; e < E; e++) ip_pair_real<RM, true>(v[r].e[e], v[q].e[e], m0, m1, m2, m3);
-        }
+        for (int e = 0; e < E; e++) ip_pair_real<RM, true>(v[r].e[e], v[q].e[e], m0, a1, m2, a3);
     }
 }
 
@@ -693,6 +812,44 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
                     }
                 }
                 break; }
+            case PO_ROT_REG: {
+                const cx<T> km = c.c0;
+                if (o.c == 1) {    // inside the complex64 vector
+                    if constexpr (E == 2) {
+#pragma unroll
+                        for (int r = 0; r < R; r++) ip_shear3(v[r].e[0], v[r].e[1], km.re, km.im);
+                    }
+                } else {
+                    pass_rot_reg_at<T, KH, R, E>(v, o.a, km.re, km.im);
+                }
+                break; }
+            case PO_ROT_LANE: {
+                const cx<T> km = c.c0;
+                const bool hi = (lane >> o.a) & 1;
+                switch (o.a) {
+                case 0: pass_rot_lane<T, 1, R, E>(v, hi, o.c, km.re); break;
+                case 1: pass_rot_lane<T, 2, R, E>(v, hi, o.c, km.re); break;
+                case 2: pass_rot_lane<T, 4, R, E>(v, hi, o.c, km.re); break;
+                case 3: pass_rot_lane<T, 8, R, E>(v, hi, o.c, km.re); break;
+                case 4: pass_rot_lane_swap<T, 16, R, E>(v, km.re, km.im); break;
+                case 5: pass_rot_lane_swap<T, 32, R, E>(v, km.re, km.im); break;
+                default: break;
+                }
+                break; }
+            case PO_SIGN: {
+                const int t0 = rec.dw(5), t1 = rec.dw(6);
+                if (t0 >= 0 && !((tb >> t0) & 1)) break;   // a tile bit clear: no amplitude of the wave
+                if (t1 >= 0 && !((tb >> t1) & 1)) break;
+                uint32_t mk[E];
+#pragma unroll
+                for (int e = 0; e < E; e++)
+                    mk[e] = (((uint32_t)(base + e) & (uint32_t)o.a) == (uint32_t)o.a) ? 0x80000000u : 0u;
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    if ((r & o.b) == o.b)
+#pragma unroll
+                        for (int e = 0; e < E; e++) flip_sign(v[r].e[e], mk[e]);
+                break; }
             case PO_PHASES: {
                 // L per lane (and vector element): the scalar times z_p for each
                 // marked position whose bit is set
@@ -813,8 +970,16 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
             }
         }
 
+        // The row addresses again from the tile's bits: kept from the loads, the
+        // 2^KH of them would hold 2^(KH+1) scalar registers through the op loop
+        // (and push other values out to vector-register lanes).
+        uint32_t ptl = (uint32_t)pt, pth = (uint32_t)(pt >> 32);
+#if defined(__HIP_DEVICE_COMPILE__)
+        asm volatile("" : "+s"(ptl), "+s"(pth));
+#endif
+        const uint64_t pt2 = (uint64_t)ptl | ((uint64_t)pth << 32);
 #pragma unroll
-        for (int r = 0; r < R; r++) pass_st<T>(row_ptr<T, KH>(s, pt, hb, r) + slot, v[r]);
+        for (int r = 0; r < R; r++) pass_st<T>(row_ptr<T, KH>(s, pt2, hb, r) + slot, v[r]);
     }
 }
 

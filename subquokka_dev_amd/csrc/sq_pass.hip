@@ -145,15 +145,19 @@ template <typename T> double executed_flops(const PassOp<T> &o) {
     // the lane factor (or the scalar alone) and the register table: 6 each;
     // forming L x t per amplitude when both are present: 6 more
     case PO_PHASES: return o.b ? (o.a ? 12.0 : 6.0) : 6.0;
+    // three shears: 6 FMAs per pair; a scaled lane rotation: one FMA per part
+    case PO_ROT_REG: return 6.0;
+    case PO_ROT_LANE: return o.a >= 4 ? 6.0 : 4.0;
     default: return 0.0;
     }
 }
 
 // A one-qubit dense gate as phase, real rotation, phase:
 //   U = g diag(1, zl) R diag(1, zr),   R = [[c, -s], [s, c]],  c, s >= 0,
-// i.e. U = g [[c, -s zr], [s zl, c zl zr]].  Applied as R alone, the gate
-// costs 8 FP64 operations per pair instead of 14-16; the phases of all qubits
-// of a pass that fall due together go into one PO_PHASES op.  ok = false when
+// i.e. U = g [[c, -s zr], [s zl, c zl zr]].  The pass applies R as three
+// shears (6 FMAs per pair) or, on lane bits 0-3, scaled to one FMA per part,
+// instead of 14-16 operations per pair for U; the phases of all qubits of a
+// pass that fall due together go into one PO_PHASES op.  ok = false when
 // the coefficients are already real, or when the product of the factors
 // differs from U by more than 1e-15 (a matrix that is not unitary).
 struct PhaseSplit {
@@ -185,6 +189,61 @@ PhaseSplit phase_split(const LoweredGate &u) {
     p.ok = true;
     p.g = g; p.zl = zl; p.zr = zr; p.c = c; p.s = s;
     return p;
+}
+
+// The rotation [[c, -s], [s, c]] (c, s >= 0, c^2 + s^2 = 1 to rounding: the
+// shears reproduce it only then) on virtual position q as an op:
+// three shears on a register slot, inside the complex64 vector and on lane
+// bits 4-5 (after the swap with register bit 0); on lane bits 0-3 the
+// rotation divided by max(c, s), which *scale is set to (the caller applies
+// it with the pass's scalar).
+template <typename T>
+PassOp<T> rotation_op(double c, double s, int q, const std::vector<int> &regs, double &scale) {
+    constexpr int VB = Traits<T>::VB;
+    constexpr int S = VB + 6;
+    PassOp<T> o{};
+    o.b = -1;
+    scale = 1.0;
+    const double k = -s / (1.0 + c);
+    o.coef[0] = {(T)k, (T)s};
+    if (q < VB) {
+        o.kind = PO_ROT_REG;
+        o.c = 1;
+    } else if (q < S) {
+        o.kind = PO_ROT_LANE;
+        o.a = q - VB;
+        if (o.a < 4) {
+            o.c = c >= s ? 0 : 1;
+            scale = o.c == 0 ? c : s;
+            o.coef[0] = {(T)(o.c == 0 ? s / c : c / s), (T)0};
+        }
+    } else {
+        o.kind = PO_ROT_REG;
+        o.a = (int)(std::find(regs.begin(), regs.end(), q) - regs.begin());
+    }
+    return o;
+}
+
+// CP(pi) / CZ as lowered: a two-qubit diagonal whose only entry other than 1
+// is -1 (within one ulp: e^{i pi} evaluates to -1 + 1.2e-16 i; leaving out
+// that term changes each amplitude by less than one ulp of its magnitude).
+bool is_sign_cp(const LoweredGate &d) {
+    return d.cls == OpClass::Diag && d.k == 2 && d.diag_active == 8u && d.m[3].real() == -1.0 &&
+           std::abs(d.m[3].imag()) <= 0x1p-52;
+}
+template <typename T> PassOp<T> sign_op(const LoweredGate &d, const std::vector<int> &regs, int S) {
+    PassOp<T> o{};
+    o.kind = PO_SIGN;
+    o.pad[0] = o.pad[1] = -1;
+    int nt = 0;
+    for (int x = 0; x < 2; x++) {
+        const int t = d.t[x];
+        const auto it = std::find(regs.begin(), regs.end(), t);
+        if (t < S) o.a |= 1 << t;
+        else if (it != regs.end()) o.b |= 1 << (int)(it - regs.begin());
+        else o.pad[nt++] = t;
+    }
+    return o;
 }
 
 // Appends a PO_PHASES op (and its coefficient records): fac[p] is the factor
@@ -349,9 +408,18 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
             lg.push_back(low[gi]);
             sq::relabel(lg.back(), vpos);
         }
+        const std::complex<double> one(1.0, 0.0);
+        std::vector<PhaseSplit> sp(lg.size());
+        bool any_split = false;
+        if (e->phase_split)
+            for (size_t i = 0; i < lg.size(); i++) {
+                sp[i] = phase_split(lg[i]);
+                any_split = any_split || sp[i].ok;
+            }
         // complex<double>: a controlled phase (diagonal, only |11> not 1) whose
         // next gate on either qubit is a dense gate on a register slot or a
         // lane bit goes into that gate's op, with the other qubit as the control
+        // (not into a split gate: its rotation runs as shears)
         std::vector<int> fold_into(lg.size(), -1), folded_from(lg.size(), -1);
         if constexpr (sizeof(T) == 8) {
             auto in_regs = [&](int t) { return std::find(regs.begin(), regs.end(), t) != regs.end(); };
@@ -363,7 +431,7 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
                     bool touches = false;
                     for (int x = 0; x < u.k; x++) touches = touches || u.t[x] == d.t[0] || u.t[x] == d.t[1];
                     if (!touches) continue;
-                    if (u.cls == OpClass::Dense && u.k == 1 && folded_from[j] < 0 &&
+                    if (u.cls == OpClass::Dense && u.k == 1 && folded_from[j] < 0 && !sp[j].ok &&
                         (in_regs(u.t[0]) || (u.t[0] >= VB && u.t[0] < S))) {
                         fold_into[i] = (int)j;
                         folded_from[j] = (int)i;
@@ -379,14 +447,6 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
         // qubit's next gate and the one-qubit diagonals before it (all of them
         // diagonal, so they commute up to that gate).  The scalars collect in
         // gpass; the pass ends with a flush of what is left.
-        const std::complex<double> one(1.0, 0.0);
-        std::vector<PhaseSplit> sp(lg.size());
-        bool any_split = false;
-        if (e->phase_split)
-            for (size_t i = 0; i < lg.size(); i++) {
-                sp[i] = phase_split(lg[i]);
-                any_split = any_split || sp[i].ok;
-            }
         auto resident = [&](int t) { return t < S || std::find(regs.begin(), regs.end(), t) != regs.end(); };
         std::vector<std::complex<double>> pend(n, one);
         std::vector<char> absorbed(lg.size(), 0), zr_done(lg.size(), 0);
@@ -433,6 +493,15 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
                 }
                 continue;
             }
+            // complex<double>: CP(pi) as a sign flip, rotations as shears (or
+            // scaled); complex<float> keeps the real 2x2 product and the
+            // diagonal op, which measured faster there (FP32 arithmetic is
+            // cheap next to the extra bit operations)
+            constexpr bool kShears = sizeof(T) == 8;
+            if (kShears && any_split && is_sign_cp(lg[i])) {   // CP(pi): a sign flip
+                ops.push_back(sign_op<T>(lg[i], regs, S));
+                continue;
+            }
             LoweredGate gi = lg[i];
             if (any_split && gi.cls == OpClass::Dense) {
                 const int q = gi.t[0];
@@ -442,13 +511,23 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
                 pend[q] = one;
                 zr_done[i] = 1;
                 if (sp[i].ok) {
-                    gi.m[0] = sp[i].c;
-                    gi.m[1] = -sp[i].s;
-                    gi.m[2] = sp[i].s;
-                    gi.m[3] = sp[i].c;
                     pend[q] = sp[i].zl;
                     gpass *= sp[i].g;
-                    if (sp[i].c == 1.0 && sp[i].s == 0.0 && folded_from[i] < 0) continue;   // R = I
+                    const double c = sp[i].c, s = sp[i].s;
+                    if (c == 1.0 && s == 0.0) continue;   // R = I
+                    if (kShears && std::abs(c * c + s * s - 1.0) <= 1e-15) {   // a rotation: shears or scaled
+                        double scale = 1.0;
+                        ops.push_back(rotation_op<T>(c, s, q, regs, scale));
+                        gpass *= scale;
+                        l.flops_per_amp += executed_flops(ops.back());
+                        continue;
+                    }
+                    // complex<float>, or a multiple of a rotation (a matrix that
+                    // is not unitary): the real 2x2 product
+                    gi.m[0] = c;
+                    gi.m[1] = -s;
+                    gi.m[2] = s;
+                    gi.m[3] = c;
                 }
             }
             encode_pass_op<T>(gi, regs, ops);
@@ -492,6 +571,9 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
         case PO_DIAG_REG1: idx = 5; break;
         case PO_DIAG_REG2: idx = 6; break;
         case PO_PHASES: idx = 7; oi += o.c; break;   // past its coefficient records
+        case PO_ROT_REG: idx = o.c == 1 ? 3 : 0; break;
+        case PO_ROT_LANE: idx = o.a < 4 ? 1 : 2; break;
+        case PO_SIGN: idx = o.b == 0 ? 4 : (__builtin_popcount((unsigned)o.b) == 1 ? 5 : 6); break;
         case PO_DENSE_REG_C:
         case PO_DENSE_LANE_C: {
             const int t = o.kind == PO_DENSE_REG_C ? 0 : (o.a < 4 ? 1 : 2);

@@ -132,16 +132,18 @@ def gate_line(kind, qs, rng_py, nrng):
 
 
 def fold_pairs_circuit(n, n_pairs, seed):
-    """CP on a random pair followed by a dense gate on one of its qubits: the
-    complex128 register passes fold most of these phases into the dense
-    gate's op, in every target/control placement at n = 14."""
+    """CP on a random pair followed by a dense gate with real coefficients (H,
+    RY, X) on one of its qubits: the complex128 register passes fold most of
+    these phases into the dense gate's op, in every target/control placement
+    at n = 14.  (A dense gate with complex coefficients is split into phases
+    and a rotation, which takes no fold.)"""
     rng = random.Random(seed)
     nrng = np.random.default_rng(seed)
     lines = []
     for _ in range(n_pairs):
         a, b = rng.sample(range(n), 2)
         lines.append(f"CP {min(a, b)} {max(a, b)} {rng.uniform(0, 6.28)!r}")
-        lines.append(gate_line(rng.choice(["U1", "H", "RX"]), [rng.choice([a, b])], rng, nrng))
+        lines.append(gate_line(rng.choice(["RY", "H", "X"]), [rng.choice([a, b])], rng, nrng))
     return "\n".join(lines) + "\n"
 
 

@@ -316,8 +316,8 @@ def test_register_pass_phase_split(emul, tmp_path, oracle, dtype, first_dense):
     # slot 7).  Split and unsplit programs against the oracle and each other,
     # with the edge cases of the split written out as matrices: a phase-only
     # gate (the rotation is the identity and no op is left), an anti-diagonal
-    # one (cos 0), a matrix that is not unitary (kept whole), and one-qubit
-    # diagonals on lane and tile qubits between them.
+    # one (cos 0), a matrix that is not unitary (kept whole), half a unitary,
+    # and one-qubit diagonals on lane and tile qubits between them.
     import math
     n = 13
     text = register_pass_circuit(n, 200, 77, first_dense)
@@ -328,6 +328,13 @@ def test_register_pass_phase_split(emul, tmp_path, oracle, dtype, first_dense):
         extra.append(f"U1 {q} 0 0 {math.cos(b)!r} {math.sin(b)!r} {math.cos(a)!r} {math.sin(a)!r} 0 0")
         extra.append(f"P {q % first_dense} {0.3 + q!r}")
         extra.append(f"U1 {q} 0.5 0.1 0.2 0.3 0.4 0.5 0.6 0.7")
+        # half a unitary: it splits, but its rotation part is not a rotation
+        # (c^2 + s^2 = 1/4), so it runs as the real 2x2 product
+        t, ph, la = 0.3 + 0.1 * q, 1.1, -0.4
+        u = [0.5 * math.cos(t), -0.5 * complex(math.cos(la), math.sin(la)) * math.sin(t),
+             0.5 * complex(math.cos(ph), math.sin(ph)) * math.sin(t),
+             0.5 * complex(math.cos(ph + la), math.sin(ph + la)) * math.cos(t)]
+        extra.append(f"U1 {q} " + " ".join(f"{complex(z).real!r} {complex(z).imag!r}" for z in u))
         extra.append(f"RZ {n - 1 - (q % 3)} {0.2 * q!r}")
     text = text + "\n".join(extra) + "\n" + text
     want = oracle.run(sq.gen_state(n, n), n, text)

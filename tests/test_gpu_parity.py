@@ -283,7 +283,7 @@ def test_register_passes_every_fold_form(oracle):
         census = a_.pass_census()
         fold = census["fold"]
         assert (fold > 0).all(), fold
-        assert census["phases"] > 0   # the U1 / RX gates run as phase, real rotation, phase
+        assert census["phases"] == 0   # real gates only: nothing is split
         assert a_.max_abs_diff(b_) <= 1e-14
         check(a_.get_state(np.complex128), oracle.run(s0, n, text), "c128", "every fold form")
     finally:
