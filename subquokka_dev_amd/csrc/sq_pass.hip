@@ -195,8 +195,8 @@ PhaseSplit phase_split(const LoweredGate &u) {
 // shears reproduce it only then) on virtual position q as an op:
 // three shears on a register slot, inside the complex64 vector and on lane
 // bits 4-5 (after the swap with register bit 0); on lane bits 0-3 the
-// rotation divided by max(c, s), which *scale is set to (the caller applies
-// it with the pass's scalar).
+// rotation divided by c (or by s when c < s / 8), which *scale is set to (the
+// caller applies it with the pass's scalar).
 template <typename T>
 PassOp<T> rotation_op(double c, double s, int q, const std::vector<int> &regs, double &scale) {
     constexpr int VB = Traits<T>::VB;
@@ -213,7 +213,10 @@ PassOp<T> rotation_op(double c, double s, int q, const std::vector<int> &regs, d
         o.kind = PO_ROT_LANE;
         o.a = q - VB;
         if (o.a < 4) {
-            o.c = c >= s ? 0 : 1;
+            // R / c (one FMA per part; its error stays eps (c|x| + s|p|)) while
+            // s / c <= 8, else R / s with the partner's sign flipped on half the
+            // lanes; the scale per gate stays above 1/8
+            o.c = 8.0 * c >= s ? 0 : 1;
             scale = o.c == 0 ? c : s;
             o.coef[0] = {(T)(o.c == 0 ? s / c : c / s), (T)0};
         }

@@ -291,6 +291,71 @@ def test_register_passes_every_fold_form(oracle):
         b_.close()
 
 
+def split_edge_circuit(n, seed):
+    # Dense gates written out as matrices on every qubit (vector, lane bits 0-3
+    # and 4-5, register slots, tile bits): U3-form rotations, a phase-only gate
+    # (its rotation is the identity), an anti-diagonal one (cos 0), Y, a matrix
+    # that is not unitary, half a unitary (its rotation part has c^2 + s^2 =
+    # 1/4), with CP(pi), P and RZ between them.
+    import math
+    import random as _random
+    from helpers import gate_line
+    rng = _random.Random(seed)
+    nrng = np.random.default_rng(seed)
+
+    def mat(q, u):
+        return f"U1 {q} " + " ".join(f"{complex(z).real!r} {complex(z).imag!r}" for z in u)
+
+    lines = []
+    for rep in range(3):
+        for q in range(n):
+            a, b = 0.3 + 0.2 * q + rep, -1.1 + 0.1 * q
+            lines.append(gate_line("U3", [q], rng, nrng))
+            lines.append(mat(q, [1, 0, 0, complex(math.cos(a), math.sin(a))]))
+            lines.append(mat(q, [0, complex(math.cos(b), math.sin(b)), complex(math.cos(a), math.sin(a)), 0]))
+            lines.append(f"Y {q}")
+            t = 0.4 + 0.05 * q
+            lines.append(mat(q, [0.5 * math.cos(t), -0.5 * complex(math.cos(b), math.sin(b)) * math.sin(t),
+                                 0.5 * complex(math.cos(a), math.sin(a)) * math.sin(t),
+                                 0.5 * complex(math.cos(a + b), math.sin(a + b)) * math.cos(t)]))
+            lines.append(mat(q, [complex(1.9, 0.1), complex(0.2, 0.3), complex(0.4, 0.5), complex(1.7, 0.7)]))
+            x, y = rng.sample(range(n), 2)
+            lines.append(f"CP {min(x, y)} {max(x, y)} 3.141592653589793")
+            lines.append(f"P {rng.randrange(n)} {rng.uniform(0, 6.28)!r}")
+            lines.append(f"RZ {rng.randrange(n)} {rng.uniform(0, 6.28)!r}")
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_register_passes_split_edge_cases(oracle, dtype):
+    # every form the split takes (shears, both scaled lane forms, sign flips,
+    # identity rotations skipped, kept-whole matrices) against the oracle and
+    # one launch per gate; the non-unitary gates scale the state, so the
+    # tolerance is relative to the largest amplitude
+    n = 14
+    text = split_edge_circuit(n, 5)
+    s0 = random_state(n, 21)
+    say(f"register passes: split edge cases {dtype}")
+    want = oracle.run(s0, n, text)
+    scale = float(np.max(np.abs(want)))
+    host = s0.astype(np.complex128 if dtype == "c128" else np.complex64)
+    a = sq.Engine(n, dtype=dtype)
+    b = sq.Engine(n, dtype=dtype, sweep_mode=1)
+    try:
+        a.set_state(host)
+        b.set_state(host)
+        a.apply_text(text)
+        b.apply_text(text)
+        got = a.get_state(np.complex128)
+        assert a.pass_census()["phases"] > 0
+        tol = (1e-12 if dtype == "c128" else 2e-6) * max(1.0, scale)
+        assert float(np.max(np.abs(got - want))) <= tol, (float(np.max(np.abs(got - want))), scale)
+        assert float(np.max(np.abs(got - b.get_state(np.complex128)))) <= tol
+    finally:
+        a.close()
+        b.close()
+
+
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("n", [10, 11, 12])
 def test_register_passes_smallest_states(oracle, dtype, n):
