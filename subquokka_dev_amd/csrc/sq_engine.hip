@@ -84,7 +84,13 @@ unsigned sq_engine::grid_for(uint64_t work_items, unsigned per_block) const {
 unsigned sq_engine::grid_flat(uint64_t work_items, unsigned per_block) const {
     uint64_t b = (work_items + per_block - 1) / per_block;
     if (b < 1) b = 1;
-    if (b > 0x7fffffffULL) b = 0x7fffffffULL;   // kernels keep a grid-stride loop
+    // A launch holds fewer than 2^32 threads (kBlock per workgroup): a
+    // one-qubit gate on a lane bit of a 32-qubit complex128 state asks for
+    // 2^24 workgroups, exactly 2^32.  Kernels launched here either keep a
+    // grid-stride loop or (register and fused passes) stay far below the cap
+    // for any state that fits in HBM.
+    const uint64_t cap = 0xffffffffULL / kBlock;
+    if (b > cap) b = cap;
     return (unsigned)b;
 }
 

@@ -316,6 +316,8 @@ double emit_phases(const std::vector<std::complex<double>> &fac, const std::vect
 template <typename T, int KH>
 int launch_pass(sq_engine *e, const PassArgs<T> &a) {
     const unsigned grid = e->grid_flat(a.n_tiles, kBlock / 64);
+    if ((uint64_t)grid * (kBlock / 64) < a.n_tiles)   // one tile per wave, no grid-stride loop
+        return e->fail(SQ_ERR_ARG, "register pass: " + std::to_string(a.n_tiles) + " wave tiles exceed one launch");
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pass<T, KH>), grid, kBlock, 0, e->stream,
                        reinterpret_cast<cx<T> *>(e->d_state), a);
     return e->check_launch("k_pass");

@@ -454,6 +454,32 @@ def test_large_state_round_trip_property():
         eng.close()
 
 
+def test_one_launch_per_gate_32q_lane_targets():
+    # 32 qubits complex128 (64 GiB per state): a one-qubit gate on a lane bit
+    # asks for 2^24 workgroups of 256 threads, 2^32 threads, past what one
+    # launch holds; the grid is capped and the kernels' grid-stride loops
+    # cover the rest.  One launch per gate against the register passes, both
+    # on the device; the norm is that of six H gates with the reference's
+    # literal 0.70710678118 (2 h^2 per gate, not exactly 1).
+    from helpers import H_LITERAL
+    n = 32
+    text = "".join(f"H {q}\n" for q in range(6)) + "RX 3 0.7\nRY 31 1.3\nCP 2 30 3.141592653589793\n" + \
+        "U1 0 0.6 0.0 0.0 0.8 0.8 0.0 0.0 -0.6\nP 5 0.4\n"
+    say("32q one launch per gate")
+    a = sq.Engine(n, dtype="c128", sweep_mode=1)
+    try:
+        a.apply_text(text)
+        assert abs(a.norm2() - (2 * H_LITERAL ** 2) ** 6) < 1e-13
+        b = sq.Engine(n, dtype="c128")
+        try:
+            b.apply_text(text)
+            assert a.max_abs_diff(b) < 1e-14
+        finally:
+            b.close()
+    finally:
+        a.close()
+
+
 def test_full_circuit_30q_c64_against_c128():
     # the whole 900-gate random-layer circuit at 30 qubits as complex64
     # register passes against complex128 (size-independent: fidelity and the
