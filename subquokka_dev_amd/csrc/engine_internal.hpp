@@ -77,6 +77,7 @@ struct sq_engine {
     int lane_moves = 3;             // top lane positions a pass may give to other qubits (0..3)
     int plan_beam = sq::kDefaultPlanBeam;   // register-pass planner: sets kept per size (1: greedy)
     bool phase_split = true;        // register passes: one-qubit dense gates as phase, real rotation, phase
+    void *d_state2 = nullptr;       // run_plain_program: a second state the same program runs on (set per call)
     // Register-pass schedules of recent gate lists, keyed by their structure
     // (qubits and gate classes: the schedule does not depend on angles or
     // coefficients), most recent first.

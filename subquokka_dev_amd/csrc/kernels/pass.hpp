@@ -152,6 +152,7 @@ template <typename T> struct PassArgs {
     int lane_pos[6];                     // slot-index bit (index bit - VB) of lane bit k, <= 27
     int reg_pos[kPassMaxKH];             // index bit of register slot j
     int tile_pos[kPassMaxTileBits];      // index bit of tile-number bit k
+    cx<T> *s2;               // a second state the same program runs on (tiles n_tiles.. 2 n_tiles - 1), or null
 };
 
 // In-place updates.  The op loop carries the state in fixed registers, so
@@ -714,9 +715,15 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
     // apart), where the dealt order ran up to 1.35x slower.
     uint64_t bid = blockIdx.x;
     if ((gridDim.x & 7) == 0) bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
-    // one tile per wave (the grid covers every tile)
-    const uint64_t t = bid * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (t >= a.n_tiles) return;
+    // one tile per wave (the grid covers every tile); with a second state,
+    // the waves past the first state's tiles take the second one's
+    uint64_t t = bid * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    cx<T> *st = s;
+    if (t >= a.n_tiles) {
+        if (!a.s2 || t >= 2 * a.n_tiles) return;
+        t -= a.n_tiles;
+        st = a.s2;
+    }
 
     // state addresses of the tile: the tile number's bits placed by tile_pos
     // (uniform), the register rows by reg_pos, the lane's 16-byte slot by
@@ -742,7 +749,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
 
         Pack<T> v[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) pass_ld<T>(row_ptr<T, KH>(s, pt, hb, r) + slot, v[r]);
+        for (int r = 0; r < R; r++) pass_ld<T>(row_ptr<T, KH>(st, pt, hb, r) + slot, v[r]);
 
         for (int op = 0; op < a.n_ops; op++) {
             // every field is read up front: one batch of scalar loads, one wait
@@ -979,7 +986,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(cx<T> *__restrict__ s, const Pa
 #endif
         const uint64_t pt2 = (uint64_t)ptl | ((uint64_t)pth << 32);
 #pragma unroll
-        for (int r = 0; r < R; r++) pass_st<T>(row_ptr<T, KH>(s, pt2, hb, r) + slot, v[r]);
+        for (int r = 0; r < R; r++) pass_st<T>(row_ptr<T, KH>(st, pt2, hb, r) + slot, v[r]);
     }
 }
 

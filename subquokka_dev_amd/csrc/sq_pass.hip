@@ -315,9 +315,10 @@ double emit_phases(const std::vector<std::complex<double>> &fac, const std::vect
 
 template <typename T, int KH>
 int launch_pass(sq_engine *e, const PassArgs<T> &a) {
-    const unsigned grid = e->grid_flat(a.n_tiles, kBlock / 64);
-    if ((uint64_t)grid * (kBlock / 64) < a.n_tiles)   // one tile per wave, no grid-stride loop
-        return e->fail(SQ_ERR_ARG, "register pass: " + std::to_string(a.n_tiles) + " wave tiles exceed one launch");
+    const uint64_t tiles = a.n_tiles * (a.s2 ? 2 : 1);
+    const unsigned grid = e->grid_flat(tiles, kBlock / 64);
+    if ((uint64_t)grid * (kBlock / 64) < tiles)   // one tile per wave, no grid-stride loop
+        return e->fail(SQ_ERR_ARG, "register pass: " + std::to_string(tiles) + " wave tiles exceed one launch");
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pass<T, KH>), grid, kBlock, 0, e->stream,
                        reinterpret_cast<cx<T> *>(e->d_state), a);
     return e->check_launch("k_pass");
@@ -624,16 +625,26 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
 }
 
 // Runs a program on the engine's current state buffer (e->n must be the
-// program's n); its op records are at prog.d_ops.
+// program's n), and on e->d_state2 too when that is set (one launch covers
+// both); its op records are at prog.d_ops.
 template <typename T>
 int run_plain_t(sq_engine *e, const PlainProgram &prog, const std::vector<LoweredGate> &low) {
     constexpr int VB = Traits<T>::VB;
     constexpr int S = VB + 6;
     const int n = e->n, kh = prog.kh;
     if (n != prog.n) return e->fail(SQ_ERR_ARG, "internal: pass program planned for another size");
+    // one launch per gate: on each state in turn
+    auto apply_one = [&](const LoweredGate &g) {
+        int rc = e->apply_lowered(g);
+        if (rc || !e->d_state2) return rc;
+        std::swap(e->d_state, e->d_state2);
+        rc = e->apply_lowered(g);
+        std::swap(e->d_state, e->d_state2);
+        return rc;
+    };
     if (prog.per_gate) {
         for (const auto &g : low) {
-            int rc = e->apply_lowered(g);
+            int rc = apply_one(g);
             if (rc) return rc;
         }
         return SQ_OK;
@@ -641,15 +652,17 @@ int run_plain_t(sq_engine *e, const PlainProgram &prog, const std::vector<Lowere
     const PassOp<T> *dops = reinterpret_cast<const PassOp<T> *>(prog.d_ops);
     char nm[48];
     std::snprintf(nm, sizeof nm, "pass%d_%s", kh, e->dtype == SQ_C128 ? "c128" : "c64");
-    const double bytes = 2.0 * e->amp_bytes() * std::ldexp(1.0, n);
+    const double vecs = e->d_state2 ? 2.0 : 1.0;
+    const double bytes = 2.0 * e->amp_bytes() * std::ldexp(1.0, n) * vecs;
     for (const auto &l : prog.launches) {
         if (!l.merged) {
-            int rc = e->apply_lowered(low[l.gate]);
+            int rc = apply_one(low[l.gate]);
             if (rc) return rc;
             continue;
         }
         if (l.n_ops == 0) continue;   // the kernel reads its first op unconditionally
         PassArgs<T> a{};
+        a.s2 = reinterpret_cast<cx<T> *>(e->d_state2);
         a.ops = dops + l.op_begin;
         a.n_ops = l.n_ops;
         a.n_tile_bits = n - S - kh;
@@ -670,7 +683,7 @@ int run_plain_t(sq_engine *e, const PlainProgram &prog, const std::vector<Lowere
             if (std::find(l.hpos, l.hpos + kh, v) != l.hpos + kh) continue;
             a.tile_pos[k++] = l.v2l[v];
         }
-        int slot = e->stat_begin(nm, bytes, l.flops_per_amp * std::ldexp(1.0, n));
+        int slot = e->stat_begin(nm, bytes, l.flops_per_amp * std::ldexp(1.0, n) * vecs);
         int rc;
         switch (kh) {
         case 1: rc = launch_pass<T, 1>(e, a); break;
