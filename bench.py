@@ -566,7 +566,9 @@ def main():
            "config": {"workload": f"random_layers_{n}q_{layers}l_seed{SEED}_plain_c128", "n_qubits": n,
                       "layers": layers, "gates": ng, "state": "complex128 in HBM",
                       "mode": "plain gate list (MEM_Runner::run(vector<Gate*>&) form) as register passes, "
-                              "5 register qubits + 6 lane qubits per pass (3 lane positions chosen per pass)",
+                              "5 register qubits + 6 lane qubits per pass (3 lane positions chosen per pass); "
+                              "each U3 as its phases (applied together per pass) and a real rotation (three "
+                              "in-place shears, or one FMA per part on lane bits 0-3), CZ as sign flips",
                       "schedule": "planned on the first (warm-up) call, reused from the engine's schedule cache "
                                   "in the timed steps; op records encoded and uploaded every step"},
            "hbm_gbs": round(hbm_gbs, 1), "roofline": roofline, "kernels_c128": kernels}
