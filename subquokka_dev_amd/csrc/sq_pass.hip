@@ -195,8 +195,8 @@ PhaseSplit phase_split(const LoweredGate &u) {
 // shears reproduce it only then) on virtual position q as an op:
 // three shears on a register slot, inside the complex64 vector and on lane
 // bits 4-5 (after the swap with register bit 0); on lane bits 0-3 the
-// rotation divided by c (or by s when c < s / 8), which *scale is set to (the
-// caller applies it with the pass's scalar).
+// rotation divided by c (or by s when c is small against s), which *scale is
+// set to (the caller applies it with the pass's scalar).
 template <typename T>
 PassOp<T> rotation_op(double c, double s, int q, const std::vector<int> &regs, double &scale) {
     constexpr int VB = Traits<T>::VB;
@@ -214,9 +214,10 @@ PassOp<T> rotation_op(double c, double s, int q, const std::vector<int> &regs, d
         o.a = q - VB;
         if (o.a < 4) {
             // R / c (one FMA per part; its error stays eps (c|x| + s|p|)) while
-            // s / c <= 8, else R / s with the partner's sign flipped on half the
-            // lanes; the scale per gate stays above 1/8
-            o.c = 8.0 * c >= s ? 0 : 1;
+            // s / c <= 8 (complex<float>: s / c <= 1), else R / s with the
+            // partner's sign flipped on half the lanes; the scale per gate stays
+            // above 1/8 (1/sqrt(2)), far from the range limits of the pass
+            o.c = (sizeof(T) == 8 ? 8.0 : 1.0) * c >= s ? 0 : 1;
             scale = o.c == 0 ? c : s;
             o.coef[0] = {(T)(o.c == 0 ? s / c : c / s), (T)0};
         }
@@ -499,12 +500,12 @@ int build_plain_t(sq_engine *e, const std::vector<sq_gate> &gates, const std::ve
                 }
                 continue;
             }
-            // complex<double>: CP(pi) as a sign flip, rotations as shears (or
-            // scaled); complex<float> keeps the real 2x2 product and the
-            // diagonal op, which measured faster there (FP32 arithmetic is
-            // cheap next to the extra bit operations)
-            constexpr bool kShears = sizeof(T) == 8;
-            if (kShears && any_split && is_sign_cp(lg[i])) {   // CP(pi): a sign flip
+            // Rotations as shears (or scaled) for both types; CP(pi) as a sign
+            // flip for complex<double> only: complex<float> keeps the diagonal
+            // op, which measured faster there (FP32 arithmetic is cheap next to
+            // the bit operations)
+            constexpr bool kShears = true, kSigns = sizeof(T) == 8;
+            if (kSigns && any_split && is_sign_cp(lg[i])) {   // CP(pi): a sign flip
                 ops.push_back(sign_op<T>(lg[i], regs, S));
                 continue;
             }
